@@ -1,0 +1,487 @@
+// The C ABI of include/btk.h.  Every entry point converts exceptions into a
+// status code and a thread-local message.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+
+#include "btk_internal.hpp"
+
+struct btk_tree {
+    std::unique_ptr<btk::Tree> t;
+};
+struct btk_list {
+    std::unique_ptr<btk::List> l;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    } catch (...) {
+        g_err = "unknown error";
+        return 1;
+    }
+}
+
+hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+btk_options opts_or_default(const btk_options *o) {
+    btk_options r;
+    std::memset(&r, 0, sizeof r);
+    if (o) r = *o;
+    return r;
+}
+
+void check_cfg(const btk_run_config *cfg, int dim) {
+    BTK_REQUIRE(cfg != nullptr, "null run config");
+    BTK_REQUIRE(cfg->dim == dim, "run config dim does not match the entry point");
+}
+
+// std::stoi / std::stod as read_namelist.cpp:25,31,37,39 use them: leading
+// white space, then a number; trailing text is ignored; no number is an error.
+int parse_int(const std::string &s) {
+    const char *b = s.c_str();
+    char *e = nullptr;
+    errno = 0;
+    long v = std::strtol(b, &e, 10);
+    if (e == b) throw btk::Error("namelist: no integer in '" + s + "'");
+    if (errno == ERANGE || v > INT32_MAX || v < INT32_MIN) throw btk::Error("namelist: integer out of range");
+    return (int)v;
+}
+
+double parse_double(const std::string &s) {
+    const char *b = s.c_str();
+    char *e = nullptr;
+    errno = 0;
+    double v = std::strtod(b, &e);
+    if (e == b) throw btk::Error("namelist: no number in '" + s + "'");
+    if (errno == ERANGE && (v == HUGE_VAL || v == -HUGE_VAL)) throw btk::Error("namelist: number out of range");
+    return v;
+}
+
+void copy_panels_host(const btk::Tree &t, std::vector<double> (&lo)[3], std::vector<double> (&hi)[3],
+                      std::vector<double> &rad, std::vector<int> &cnt, std::vector<int> &par,
+                      std::vector<int> &c0, std::vector<int> &lvl) {
+    const int pc = t.pc;
+    for (int d = 0; d < 3; d++) {
+        lo[d].resize(pc);
+        hi[d].resize(pc);
+        BTK_HIP(hipMemcpy(lo[d].data(), t.lo[d].p, sizeof(double) * pc, hipMemcpyDeviceToHost));
+        BTK_HIP(hipMemcpy(hi[d].data(), t.hi[d].p, sizeof(double) * pc, hipMemcpyDeviceToHost));
+    }
+    rad.resize(pc);
+    cnt.resize(pc);
+    par.resize(pc);
+    c0.resize(pc);
+    lvl.resize(pc);
+    BTK_HIP(hipMemcpy(rad.data(), t.radius.p, sizeof(double) * pc, hipMemcpyDeviceToHost));
+    BTK_HIP(hipMemcpy(cnt.data(), t.count.p, sizeof(int) * pc, hipMemcpyDeviceToHost));
+    BTK_HIP(hipMemcpy(par.data(), t.parent.p, sizeof(int) * pc, hipMemcpyDeviceToHost));
+    BTK_HIP(hipMemcpy(c0.data(), t.child0.p, sizeof(int) * pc, hipMemcpyDeviceToHost));
+    BTK_HIP(hipMemcpy(lvl.data(), t.level.p, sizeof(int) * pc, hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+extern "C" {
+
+int btk_components(int dim, int kernel) { return btk::components(dim, kernel); }
+
+const char *btk_last_error(void) { return g_err.c_str(); }
+
+int btk_release_cached(void) {
+    return guarded([&] { btk::release_cached(); });
+}
+
+// read_namelist.cpp:10-44.  The parse reads key=value lines and stops at the
+// first line whose key is not one of the four; the value text of a line
+// without '=' is the previous line's (as getline leaves it there).
+int btk_read_run_config(const char *path, btk_run_config *cfg) {
+    return guarded([&] {
+        BTK_REQUIRE(path && cfg, "null argument");
+        std::ifstream f(path);
+        if (f.fail()) throw btk::Error(std::string("namelist not found: ") + path);
+        std::string line, value;
+        for (;;) {
+            if (!std::getline(f, line)) line.clear();
+            std::string key = line;
+            const size_t eq = line.find('=');
+            if (eq != std::string::npos) {
+                key = line.substr(0, eq);
+                value = line.substr(eq + 1);
+            }
+            if (key == "interp_degree") {
+                int v = parse_int(value);
+                if (v > BTK_MAX_DEGREE) v = BTK_MAX_DEGREE;
+                cfg->interp_degree = v;
+                cfg->interp_point_count = btk::ipow(v + 1, cfg->dim);
+            } else if (key == "cluster_size") {
+                cfg->fmm_cluster_thresh = parse_int(value);
+                if (cfg->fmm_cluster_thresh <= 0)
+                    cfg->fmm_cluster_thresh = btk::ipow(2, cfg->dim) * cfg->interp_point_count;
+            } else if (key == "theta") {
+                cfg->fmm_theta = parse_double(value);
+            } else if (key == "kernel_eps") {
+                cfg->ker_eps = parse_double(value);
+            } else {
+                break;
+            }
+        }
+    });
+}
+
+int btk_tree_construction_2d(const btk_run_config *cfg, const double *d_x, const double *d_y, int n, void *stream,
+                             btk_tree **tree) {
+    return guarded([&] {
+        check_cfg(cfg, 2);
+        BTK_REQUIRE(tree, "null tree out-pointer");
+        auto h = std::make_unique<btk_tree>();
+        h->t = btk::build_tree(2, cfg->fmm_cluster_thresh, d_x, d_y, nullptr, n, as_stream(stream));
+        *tree = h.release();
+    });
+}
+
+int btk_tree_construction_3d(const btk_run_config *cfg, const double *d_x, const double *d_y, const double *d_z,
+                             int n, void *stream, btk_tree **tree) {
+    return guarded([&] {
+        check_cfg(cfg, 3);
+        BTK_REQUIRE(tree, "null tree out-pointer");
+        auto h = std::make_unique<btk_tree>();
+        h->t = btk::build_tree(3, cfg->fmm_cluster_thresh, d_x, d_y, d_z, n, as_stream(stream));
+        *tree = h.release();
+    });
+}
+
+int btk_tree_get_info(const btk_tree *tree, btk_tree_info *info) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && info, "null argument");
+        info->panel_count = tree->t->pc;
+        info->levels = tree->t->levels;
+        info->level_start = tree->t->level_start.data();
+    });
+}
+
+int btk_tree_copy_panels_2d(const btk_tree *tree, btk_panel_2d *panels) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && panels, "null argument");
+        const btk::Tree &t = *tree->t;
+        BTK_REQUIRE(t.dim == 2, "tree is not 2D");
+        BTK_HIP(hipDeviceSynchronize());
+        std::vector<double> lo[3], hi[3], rad;
+        std::vector<int> cnt, par, c0, lvl;
+        copy_panels_host(t, lo, hi, rad, cnt, par, c0, lvl);
+        for (int p = 0; p < t.pc; p++) {
+            btk_panel_2d q;
+            std::memset(&q, 0, sizeof q);
+            q.id = p;
+            q.level = lvl[p];
+            q.parent_id = par[p];
+            const int b = c0[p];
+            q.child1 = b < 0 ? -1 : b;
+            q.child2 = b < 0 ? -1 : b + 1;
+            q.child3 = b < 0 ? -1 : b + 2;
+            q.child4 = b < 0 ? -1 : b + 3;
+            q.is_leaf = b < 0;
+            q.max_x = hi[0][p];
+            q.min_x = lo[0][p];
+            q.max_y = hi[1][p];
+            q.min_y = lo[1][p];
+            q.radius = rad[p];
+            q.point_count = cnt[p];
+            panels[p] = q;
+        }
+    });
+}
+
+int btk_tree_copy_panels_3d(const btk_tree *tree, btk_panel_3d *panels) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && panels, "null argument");
+        const btk::Tree &t = *tree->t;
+        BTK_REQUIRE(t.dim == 3, "tree is not 3D");
+        BTK_HIP(hipDeviceSynchronize());
+        std::vector<double> lo[3], hi[3], rad;
+        std::vector<int> cnt, par, c0, lvl;
+        copy_panels_host(t, lo, hi, rad, cnt, par, c0, lvl);
+        for (int p = 0; p < t.pc; p++) {
+            btk_panel_3d q;
+            std::memset(&q, 0, sizeof q);
+            q.id = p;
+            q.level = lvl[p];
+            q.parent_id = par[p];
+            const int b = c0[p];
+            int *ch[8] = {&q.child1, &q.child2, &q.child3, &q.child4, &q.child5, &q.child6, &q.child7, &q.child8};
+            for (int c = 0; c < 8; c++) *ch[c] = b < 0 ? -1 : b + c;
+            q.is_leaf = b < 0;
+            q.max_x = hi[0][p];
+            q.min_x = lo[0][p];
+            q.max_y = hi[1][p];
+            q.min_y = lo[1][p];
+            q.min_z = lo[2][p];
+            q.max_z = hi[2][p];
+            q.radius = rad[p];
+            q.point_count = cnt[p];
+            panels[p] = q;
+        }
+    });
+}
+
+int btk_tree_copy_point_leaf_panel(const btk_tree *tree, int *point_leaf_panel) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && point_leaf_panel, "null argument");
+        const btk::Tree &t = *tree->t;
+        BTK_HIP(hipDeviceSynchronize());
+        if (t.n) BTK_HIP(hipMemcpy(point_leaf_panel, t.point_leaf.p, sizeof(int) * t.n, hipMemcpyDeviceToHost));
+    });
+}
+
+int btk_tree_copy_permutation(const btk_tree *tree, int *perm, int *start) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && perm && start, "null argument");
+        const btk::Tree &t = *tree->t;
+        BTK_HIP(hipDeviceSynchronize());
+        if (t.n) BTK_HIP(hipMemcpy(perm, t.perm.p, sizeof(int) * t.n, hipMemcpyDeviceToHost));
+        BTK_HIP(hipMemcpy(start, t.start.p, sizeof(int) * t.pc, hipMemcpyDeviceToHost));
+        start[t.pc] = t.n;
+    });
+}
+
+int btk_tree_copy_panel_points_inside(const btk_tree *tree, int width, int *out) {
+    return guarded([&] {
+        BTK_REQUIRE(tree && out, "null argument");
+        BTK_REQUIRE(width > 0, "width must be positive");
+        const btk::Tree &t = *tree->t;
+        BTK_HIP(hipDeviceSynchronize());
+        std::vector<int> perm(t.n), st(t.pc), cnt(t.pc), c0(t.pc);
+        if (t.n) BTK_HIP(hipMemcpy(perm.data(), t.perm.p, sizeof(int) * t.n, hipMemcpyDeviceToHost));
+        BTK_HIP(hipMemcpy(st.data(), t.start.p, sizeof(int) * t.pc, hipMemcpyDeviceToHost));
+        BTK_HIP(hipMemcpy(cnt.data(), t.count.p, sizeof(int) * t.pc, hipMemcpyDeviceToHost));
+        BTK_HIP(hipMemcpy(c0.data(), t.child0.p, sizeof(int) * t.pc, hipMemcpyDeviceToHost));
+        std::memset(out, 0, sizeof(int) * (size_t)t.pc * width);
+        for (int p = 0; p < t.pc; p++) {
+            if (c0[p] >= 0) continue;
+            BTK_REQUIRE(cnt[p] <= width, "a leaf holds more points than the row width");
+            for (int j = 0; j < cnt[p]; j++) out[(size_t)p * width + j] = perm[st[p] + j];
+        }
+    });
+}
+
+void btk_tree_destroy(btk_tree *tree) { delete tree; }
+
+static int traversal(btk_run_config *cfg, int dim, const btk_tree *target, const btk_tree *source, void *stream,
+                     btk_list **list) {
+    return guarded([&] {
+        check_cfg(cfg, dim);
+        BTK_REQUIRE(target && source && list, "null argument");
+        BTK_REQUIRE(target->t->dim == dim && source->t->dim == dim, "tree dimension mismatch");
+        auto h = std::make_unique<btk_list>();
+        h->l = btk::build_list(cfg, *target->t, *source->t, as_stream(stream));
+        *list = h.release();
+    });
+}
+
+int btk_dual_tree_traversal_2d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
+                               btk_list **list) {
+    return traversal(cfg, 2, target, source, stream, list);
+}
+
+int btk_dual_tree_traversal_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
+                               btk_list **list) {
+    return traversal(cfg, 3, target, source, stream, list);
+}
+
+int btk_list_count(const btk_list *list) { return list ? list->l->count : -1; }
+
+int btk_list_copy(const btk_list *list, btk_interact_pair *pairs) {
+    return guarded([&] {
+        BTK_REQUIRE(list && pairs, "null argument");
+        BTK_HIP(hipDeviceSynchronize());
+        if (list->l->count)
+            BTK_HIP(hipMemcpy(pairs, list->l->pairs.p, sizeof(btk_interact_pair) * list->l->count,
+                              hipMemcpyDeviceToHost));
+    });
+}
+
+// interaction_list.cpp:252-278
+int btk_split_interactions(const btk_list *list, btk_interact_pair *pp, btk_interact_pair *pc, btk_interact_pair *cp,
+                           btk_interact_pair *cc) {
+    return guarded([&] {
+        BTK_REQUIRE(list, "null list");
+        const btk::List &l = *list->l;
+        std::vector<btk_interact_pair> all(l.count);
+        BTK_HIP(hipDeviceSynchronize());
+        if (l.count)
+            BTK_HIP(hipMemcpy(all.data(), l.pairs.p, sizeof(btk_interact_pair) * l.count, hipMemcpyDeviceToHost));
+        btk_interact_pair *dst[4] = {pp, pc, cp, cc};
+        long long at[4] = {0, 0, 0, 0};
+        for (const auto &p : all) {
+            BTK_REQUIRE(p.interact_type >= 0 && p.interact_type < 4, "bad interaction type");
+            if (dst[p.interact_type]) dst[p.interact_type][at[p.interact_type]] = p;
+            at[p.interact_type]++;
+        }
+    });
+}
+
+void btk_list_destroy(btk_list *list) { delete list; }
+
+static int upward(const btk_run_config *cfg, int dim, const btk_tree *source, const double *d_charges, double *d_W,
+                  void *stream) {
+    return guarded([&] {
+        check_cfg(cfg, dim);
+        BTK_REQUIRE(source && source->t->dim == dim, "bad source tree");
+        const btk::Tree &t = *source->t;
+        hipStream_t s = as_stream(stream);
+        btk::DBuf<double> q(t.n ? t.n : 1);
+        btk::gather_sorted(t, d_charges, q.p, s);
+        btk::upward_pass(*cfg, t, q.p, d_W, s);
+        BTK_HIP(hipStreamSynchronize(s));  // q is released on return
+    });
+}
+
+int btk_upward_pass_2d(const btk_run_config *cfg, const btk_tree *source, const double *d_charges, double *d_W,
+                       void *stream) {
+    return upward(cfg, 2, source, d_charges, d_W, stream);
+}
+
+int btk_upward_pass_3d(const btk_run_config *cfg, const btk_tree *source, const double *d_charges, double *d_W,
+                       void *stream) {
+    return upward(cfg, 3, source, d_charges, d_W, stream);
+}
+
+static int interactions(const btk_run_config *cfg, int dim, const btk_options *opts, const btk_tree *target,
+                        const btk_tree *source, const btk_list *list, const double *d_charges, const double *d_W,
+                        double *d_soln, double *d_Psi, void *stream) {
+    return guarded([&] {
+        check_cfg(cfg, dim);
+        BTK_REQUIRE(target && source && list, "null argument");
+        BTK_REQUIRE(target->t->dim == dim && source->t->dim == dim, "tree dimension mismatch");
+        const btk_options o = opts_or_default(opts);
+        hipStream_t s = as_stream(stream);
+        const btk::Tree &ts = *source->t;
+        btk::DBuf<double> q(ts.n ? ts.n : 1);
+        btk::gather_sorted(ts, d_charges, q.p, s);
+        btk::fmm_interactions(*cfg, o, *target->t, ts, *list->l, q.p, d_W, d_soln, d_Psi, s);
+        BTK_HIP(hipStreamSynchronize(s));  // poisson_fmm.cpp:25 fences
+    });
+}
+
+int btk_fmm_interactions_2d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                            const btk_tree *source, const btk_list *list, const double *d_charges, const double *d_W,
+                            double *d_soln, double *d_Psi, void *stream) {
+    return interactions(cfg, 2, opts, target, source, list, d_charges, d_W, d_soln, d_Psi, stream);
+}
+
+int btk_fmm_interactions_3d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                            const btk_tree *source, const btk_list *list, const double *d_charges, const double *d_W,
+                            double *d_soln, double *d_Psi, void *stream) {
+    return interactions(cfg, 3, opts, target, source, list, d_charges, d_W, d_soln, d_Psi, stream);
+}
+
+static int downward(const btk_run_config *cfg, int dim, const btk_options *opts, const btk_tree *target,
+                    double *d_Psi, double *d_soln, void *stream) {
+    return guarded([&] {
+        check_cfg(cfg, dim);
+        BTK_REQUIRE(target && target->t->dim == dim, "bad target tree");
+        const btk_options o = opts_or_default(opts);
+        btk::downward_pass(*cfg, o, *target->t, d_Psi, d_soln, as_stream(stream));
+    });
+}
+
+int btk_downward_pass_2d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target, double *d_Psi,
+                         double *d_soln, void *stream) {
+    return downward(cfg, 2, opts, target, d_Psi, d_soln, stream);
+}
+
+int btk_downward_pass_3d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target, double *d_Psi,
+                         double *d_soln, void *stream) {
+    return downward(cfg, 3, opts, target, d_Psi, d_soln, stream);
+}
+
+int btk_direct_sum_2d(const btk_options *opts, double eps, int m, const int *d_idx, const double *d_xt,
+                      const double *d_yt, int n, const double *d_xs, const double *d_ys, const double *d_q,
+                      double *d_out, void *stream) {
+    return guarded([&] {
+        const btk_options o = opts_or_default(opts);
+        btk::direct_sum(2, o, eps, m, d_idx, d_xt, d_yt, nullptr, n, d_xs, d_ys, nullptr, d_q, d_out,
+                        as_stream(stream));
+    });
+}
+
+int btk_direct_sum_3d(const btk_options *opts, double eps, int m, const int *d_idx, const double *d_xt,
+                      const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                      const double *d_zs, const double *d_q, double *d_out, void *stream) {
+    return guarded([&] {
+        const btk_options o = opts_or_default(opts);
+        btk::direct_sum(3, o, eps, m, d_idx, d_xt, d_yt, d_zt, n, d_xs, d_ys, d_zs, d_q, d_out, as_stream(stream));
+    });
+}
+
+// example_poisson_3d.cpp:70-160 as one call; phase times by host clock with
+// the stream drained at every phase boundary.
+int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
+              const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
+              const double *d_q, int coincident, double *d_out, btk_solve_stats *stats, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in, "null run config");
+        btk_run_config cfg = *cfg_in;
+        const int dim = cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        const btk_options o = opts_or_default(opts);
+        hipStream_t s = as_stream(stream);
+        using clk = std::chrono::steady_clock;
+        auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto t0 = clk::now();
+        auto ts = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
+        std::unique_ptr<btk::Tree> tt_own;
+        if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
+        const btk::Tree &tt = coincident ? *ts : *tt_own;
+        BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t1 = clk::now();
+        auto l = btk::build_list(&cfg, tt, *ts, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t2 = clk::now();
+        const int P = btk::ipow(cfg.interp_degree + 1, dim);
+        const int C = btk::components(dim, o.kernel);
+        btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P);
+        btk::gather_sorted(*ts, d_q, q.p, s);
+        btk::upward_pass(cfg, *ts, q.p, W.p, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t3 = clk::now();
+        BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
+        btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, d_out, Psi.p, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t4 = clk::now();
+        btk::downward_pass(cfg, o, tt, Psi.p, d_out, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t5 = clk::now();
+        if (stats) {
+            stats->t_tree = sec(t0, t1);
+            stats->t_traversal = sec(t1, t2);
+            stats->t_upward = sec(t2, t3);
+            stats->t_interact = sec(t3, t4);
+            stats->t_downward = sec(t4, t5);
+            stats->t_total = sec(t0, t5);
+            for (int k = 0; k < 4; k++) {
+                stats->count[k] = l->counts[k];
+                stats->evals[k] = l->evals[k];
+            }
+            stats->panel_count_target = tt.pc;
+            stats->panel_count_source = ts->pc;
+            stats->levels_target = tt.levels;
+            stats->levels_source = ts->levels;
+        }
+    });
+}
+
+}  // extern "C"

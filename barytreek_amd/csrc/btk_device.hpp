@@ -1,0 +1,141 @@
+// Device-side building blocks: barycentric basis, kernel evaluations.
+//
+// The library is compiled with -ffp-contract=off, so every expression below
+// rounds exactly as written; fused multiply-adds are spelled out with fma()
+// where a kernel wants them and the reference's rounding does not matter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "btk_internal.hpp"
+
+namespace btk {
+
+constexpr int kMaxN1 = BTK_MAX_DEGREE + 1;
+constexpr int kWave = 64;
+
+// Barycentric Lagrange basis of degree n1-1 on the Chebyshev-Lobatto nodes of
+// [lo, hi] at x (bli_impl.hpp:8-63): nodes ta[i]*r + o, weights 1/2, (-1)^i,
+// (-1)^n/2, a node within DBL_EPSILON of x gives the unit vector.
+template <int N1>
+__device__ __forceinline__ void bli_basis(double x, double lo, double hi, const double *ta, int n1, double *out) {
+    const double r = 0.5 * (hi - lo);
+    const double o = 0.5 * (hi + lo);
+    double c[N1];
+    int hit = -1;
+#pragma unroll
+    for (int i = 0; i < N1; i++) {
+        if (i < n1) {
+            c[i] = ta[i] * r + o;
+            if (hit < 0 && fabs(x - c[i]) < DBL_EPSILON) hit = i;
+        }
+    }
+    if (hit >= 0) {
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+            if (i < n1) out[i] = (i == hit) ? 1.0 : 0.0;
+        return;
+    }
+    double den = 0.0;
+#pragma unroll
+    for (int i = 0; i < N1; i++) {
+        if (i < n1) {
+            const double sgn = (i & 1) ? -1.0 : 1.0;
+            const double w = (i == 0) ? 0.5 : ((i == n1 - 1) ? 0.5 * sgn : sgn);
+            const double v = w / (x - c[i]);
+            out[i] = v;
+            den += v;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+        if (i < n1) out[i] = out[i] / den;
+}
+
+// Mapped proxy node i of [lo, hi] (bli_impl.hpp:65-77): tb[i]*r + o.
+__device__ __forceinline__ double proxy_node(double tb_i, double lo, double hi) {
+    const double r = 0.5 * (hi - lo);
+    const double o = 0.5 * (hi + lo);
+    return tb_i * r + o;
+}
+
+// 1/sqrt(x) in FP64: hardware estimate refined by two Newton steps
+// (relative error a few units in the last place).
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x, y * y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x, y * y, 1.0);
+    y = fma(0.5 * y, e, y);
+    return y;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel evaluations.  acc[] collects the sums without the constant factor;
+// finish() applies it.  Components:
+//   POT        acc0 = sum q G0          (3D G0 = 1/R^(1/2), 2D G0 = log R)
+//   REF_VEL3D  acc  = sum q d / R       (3 components)
+//   POT_GRAD   acc0 as POT; acc1.. = sum q d / R^(3/2) (3D), sum q d / R (2D)
+// with R = r^2 + eps^2 and d = t - s.  A pair is skipped when excl is set
+// and d = 0.
+// ---------------------------------------------------------------------------
+template <int DIM, int KERNEL>
+struct KernelTraits {
+    static constexpr int C = (KERNEL == BTK_KERNEL_POT) ? 1 : (KERNEL == BTK_KERNEL_REF_VEL3D ? 3 : 1 + DIM);
+};
+
+template <int DIM, int KERNEL>
+__device__ __forceinline__ void eval_f64(double tx, double ty, double tz, double sx, double sy, double sz, double q,
+                                         double eps2, int excl, double *acc) {
+    const double dx = tx - sx, dy = ty - sy;
+    if constexpr (DIM == 3) {
+        const double dz = tz - sz;
+        const double R = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+        const bool skip = excl && dx == 0.0 && dy == 0.0 && dz == 0.0;
+        if constexpr (KERNEL == BTK_KERNEL_POT) {
+            const double inv = skip ? 0.0 : rsqrt_f64(R);
+            acc[0] = fma(q, inv, acc[0]);
+        } else if constexpr (KERNEL == BTK_KERNEL_REF_VEL3D) {
+            const double g = skip ? 0.0 : q / R;
+            acc[0] = fma(dx, g, acc[0]);
+            acc[1] = fma(dy, g, acc[1]);
+            acc[2] = fma(dz, g, acc[2]);
+        } else {
+            const double inv = skip ? 0.0 : rsqrt_f64(R);
+            const double qi = q * inv;
+            acc[0] += qi;
+            const double h = qi * inv * inv;
+            acc[1] = fma(h, dx, acc[1]);
+            acc[2] = fma(h, dy, acc[2]);
+            acc[3] = fma(h, dz, acc[3]);
+        }
+    } else {
+        (void)tz;
+        (void)sz;
+        const double R = fma(dx, dx, fma(dy, dy, eps2));
+        const bool skip = excl && dx == 0.0 && dy == 0.0;
+        const double lg = skip ? 0.0 : log(R);
+        acc[0] = fma(q, lg, acc[0]);
+        if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
+            const double g = skip ? 0.0 : q / R;
+            acc[1] = fma(g, dx, acc[1]);
+            acc[2] = fma(g, dy, acc[2]);
+        }
+    }
+}
+
+// Constant factors of each component (see eval_f64).
+template <int DIM, int KERNEL>
+__device__ __forceinline__ double comp_scale(int c, double gfc) {
+    if constexpr (DIM == 3) {
+        if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) return c == 0 ? gfc : -gfc;
+        return gfc;
+    } else {
+        if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) return c == 0 ? gfc : 2.0 * gfc;
+        return gfc;
+    }
+}
+
+}  // namespace btk

@@ -1,0 +1,188 @@
+// Internal declarations shared by the HIP translation units of libbtk.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/btk.h"
+
+namespace btk {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define BTK_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::btk::Error(std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" +   \
+                               __FILE__ + ":" + std::to_string(__LINE__) + ")");                   \
+    } while (0)
+
+#define BTK_REQUIRE(cond, msg)                                 \
+    do {                                                       \
+        if (!(cond)) throw ::btk::Error(std::string(msg));     \
+    } while (0)
+
+#define BTK_LAUNCH_CHECK() BTK_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Device memory: a size-bucketed cache in front of hipMalloc, so that
+// repeated solves of the same shape allocate nothing after the first.
+// ---------------------------------------------------------------------------
+void *dmalloc(size_t bytes);
+void dfree(void *p);
+void release_cached();
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { resize(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            reset();
+            p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) dfree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // discards contents
+    void resize(size_t count) {
+        if (count == n && p) return;
+        reset();
+        if (count) p = static_cast<T *>(dmalloc(count * sizeof(T)));
+        n = count;
+    }
+    // keeps contents (first min(n, count) elements)
+    void grow(size_t count, hipStream_t s) {
+        if (count <= n) return;
+        T *q = static_cast<T *>(dmalloc(count * sizeof(T)));
+        if (p && n) BTK_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        BTK_HIP(hipStreamSynchronize(s));
+        reset();
+        p = q;
+        n = count;
+    }
+    T *get() const { return p; }
+};
+
+// Small pinned host staging buffer for scalar read-backs.
+struct HostScratch {
+    long long *v = nullptr;
+    HostScratch();
+    ~HostScratch();
+};
+HostScratch &host_scratch();
+
+// ---------------------------------------------------------------------------
+// Device-side scans and sorts (rocPRIM/hipCUB), temp storage from the cache.
+// ---------------------------------------------------------------------------
+void inclusive_sum_i64(const long long *in, long long *out, int n, hipStream_t s);
+void inclusive_sum_i32(const int *in, int *out, int n, hipStream_t s);
+void exclusive_sum_i32(const int *in, int *out, int n, hipStream_t s);
+// Stable sort of (key, value) pairs by key; keys are < (1 << key_bits).
+void sort_pairs_i32(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int n, int key_bits,
+                    hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Tree: panels as SoA in HBM, points sorted leaf by leaf.
+// ---------------------------------------------------------------------------
+struct Tree {
+    int dim = 3;
+    int n = 0;
+    int thresh = 0;
+    int pc = 0;      // panel count
+    int levels = 0;
+    int cap = 0;     // panel capacity of the device arrays
+    std::vector<int> level_start;  // host copy, [levels + 1]
+    DBuf<double> lo[3], hi[3];
+    DBuf<double> radius;
+    DBuf<int> count, start, parent, child0, level;
+    DBuf<int> perm;        // sorted position -> original point index
+    DBuf<int> point_leaf;  // original point index -> leaf panel id
+    DBuf<double> sx, sy, sz;  // coordinates in sorted order
+    // leaves, by level-major id order, for per-leaf kernels
+    DBuf<int> leaves;
+    int n_leaves = 0;
+    int n_leaves_deepest = 0;  // the last n_leaves_deepest entries of `leaves` are on the deepest level
+    int max_leaf_count = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Interaction list in FIFO order, plus its target-grouped forms.
+// ---------------------------------------------------------------------------
+struct List {
+    int dim = 3;
+    int count = 0;
+    long long counts[4] = {0, 0, 0, 0};
+    DBuf<btk_interact_pair> pairs;  // FIFO order
+    // Entries grouped by target panel, stable in FIFO order.  "Near" entries
+    // (PP, PC: the targets are points of a target leaf) of panel p are
+    // [goff[p], goff[p+1]); "far" entries (CP, CC: the targets are the proxy
+    // points of a target cluster) of panel p are [goff[pct+p], goff[pct+p+1]).
+    int pct = 0;
+    DBuf<int> goff;             // [2*pct + 1]
+    DBuf<int> gsrc;             // source panel of each entry
+    DBuf<unsigned char> gtype;  // interact_type of each entry
+    double evals[4] = {0, 0, 0, 0};  // kernel evaluations per type
+};
+
+// Interpolation tables (host-computed with libm, so that the nodes are the
+// same bits as the reference's cos() calls).
+struct NodeTables {
+    double a[BTK_MAX_DEGREE + 1];  // cos(pi * i / n)  (bli_impl.hpp:25)
+    double b[BTK_MAX_DEGREE + 1];  // cos(pi / n * i)  (bli_impl.hpp:74)
+};
+NodeTables node_tables(int n);
+
+// Kernel-side parameters.
+struct KParams {
+    int dim;
+    int kernel;
+    int excl;
+    double eps2;
+    double gfc;
+};
+KParams make_kparams(int dim, double eps, const btk_options &o);
+int components(int dim, int kernel);
+
+// Phases.
+std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const double *y, const double *z, int n,
+                                 hipStream_t s);
+std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &t, const Tree &s, hipStream_t st);
+void upward_pass(const btk_run_config &cfg, const Tree &src, const double *q_sorted, double *W, hipStream_t s);
+void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_t s);
+void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
+                      const List &l, const double *q_sorted, const double *W, double *out, double *Psi,
+                      hipStream_t s);
+void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
+                   hipStream_t s);
+void direct_sum(int dim, const btk_options &o, double eps, int m, const int *idx, const double *xt,
+                const double *yt, const double *zt, int n, const double *xs, const double *ys,
+                const double *zs, const double *q, double *out, hipStream_t s);
+
+inline int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; i++) r *= b;
+    return r;
+}
+
+}  // namespace btk

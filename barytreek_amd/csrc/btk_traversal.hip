@@ -1,0 +1,248 @@
+// Dual-tree traversal on the GPU (the reference's dual_tree_traversal_2d/3d,
+// interaction_list.cpp:5-250, is a host-serial FIFO walk).
+//
+// The FIFO walk visits pairs generation by generation: the pairs pushed while
+// generation k is processed are exactly generation k+1, in order.  So each
+// generation is one array; a classify kernel applies the MAC and the refine
+// rule to every pair, a scan gives every emitted pair its slot in the output
+// list and every refined pair the slots of its 2^dim children in the next
+// generation, and the output list is the reference's list in the reference's
+// order.  The MAC geometry (box centres, distance, ratio) is evaluated with
+// the reference's expressions in FP64 without contraction (-ffp-contract=off).
+#include "btk_device.hpp"
+
+namespace btk {
+namespace {
+
+constexpr int TB = 256;
+
+inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
+
+struct TreeView {
+    const double *lo0, *lo1, *lo2, *hi0, *hi1, *hi2, *radius;
+    const int *count, *child0;
+};
+
+TreeView view(const Tree &t) {
+    return TreeView{t.lo[0].p, t.lo[1].p, t.lo[2].p, t.hi[0].p, t.hi[1].p, t.hi[2].p,
+                    t.radius.p, t.count.p, t.child0.p};
+}
+
+// code: 0..3 emit with that type, 4 refine target, 5 refine source
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__ cur, TreeView T, TreeView S,
+                                                 double theta, int thr, unsigned char *__restrict__ code,
+                                                 long long *__restrict__ val) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= G) return;
+    const int it = cur[i].x, is = cur[i].y;
+    // interaction_list.cpp:141-154
+    const double x1 = 0.5 * (T.lo0[it] + T.hi0[it]);
+    const double y1 = 0.5 * (T.lo1[it] + T.hi1[it]);
+    const double x2 = 0.5 * (S.lo0[is] + S.hi0[is]);
+    const double y2 = 0.5 * (S.lo1[is] + S.hi1[is]);
+    const double ex = x2 - x1, ey = y2 - y1;
+    double dist;
+    if constexpr (DIM == 3) {
+        const double z1 = 0.5 * (T.lo2[it] + T.hi2[it]);
+        const double z2 = 0.5 * (S.lo2[is] + S.hi2[is]);
+        const double ez = z2 - z1;
+        dist = sqrt(ex * ex + ey * ey + ez * ez);
+    } else {
+        dist = sqrt(ex * ex + ey * ey);
+    }
+    double sep = 100;
+    if (dist > 0) sep = (T.radius[it] + S.radius[is]) / dist;
+    const int ct = T.count[it], cs = S.count[is];
+    int c;
+    if (sep < theta) {  // :155-179
+        c = (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);
+    } else {  // :180-203
+        const bool lt = T.child0[it] < 0, lsf = S.child0[is] < 0;
+        if (lt && lsf)
+            c = 0;
+        else if (lt)
+            c = 5;
+        else if (lsf)
+            c = 4;
+        else
+            c = ct > cs ? 4 : 5;
+    }
+    code[i] = (unsigned char)c;
+    val[i] = c < 4 ? 1ll : (1ll << 32);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_write(int G, const int2 *__restrict__ cur, const unsigned char *__restrict__ code,
+            const long long *__restrict__ val, const long long *__restrict__ inc, TreeView T, TreeView S, int lbase,
+            btk_interact_pair *__restrict__ out, int2 *__restrict__ next) {
+    constexpr int NC = DIM == 3 ? 8 : 4;
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= G) return;
+    const long long ex = inc[i] - val[i];
+    const int c = code[i];
+    const int it = cur[i].x, is = cur[i].y;
+    if (c < 4) {
+        btk_interact_pair p;
+        p.target_panel = it;
+        p.source_panel = is;
+        p.interact_type = c;
+        out[lbase + (int)(ex & 0xffffffffLL)] = p;
+    } else {
+        const int k = (int)(ex >> 32);
+        // :204-238: children pushed in c1..c8 order with the other side repeated
+        if (c == 4) {
+            const int b = T.child0[it];
+            for (int j = 0; j < NC; j++) next[NC * k + j] = make_int2(b + j, is);
+        } else {
+            const int b = S.child0[is];
+            for (int j = 0; j < NC; j++) next[NC * k + j] = make_int2(it, b + j);
+        }
+    }
+}
+
+// Keys for the grouped forms: (far << tbits) | target.
+__global__ void k_group_keys(int L, const btk_interact_pair *__restrict__ pairs, int tbits, int pct,
+                             int *__restrict__ keys, int *__restrict__ vals, int *__restrict__ cnt,
+                             const int *__restrict__ tcount, const int *__restrict__ scount, int P,
+                             double *__restrict__ evals) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    __shared__ double se[4];
+    if (threadIdx.x < 4) se[threadIdx.x] = 0.0;
+    __syncthreads();
+    if (i < L) {
+        const btk_interact_pair p = pairs[i];
+        const int far = p.interact_type >= 2 ? 1 : 0;
+        keys[i] = (far << tbits) | p.target_panel;
+        vals[i] = i;
+        atomicAdd(&cnt[far * pct + p.target_panel], 1);
+        const double nt = p.interact_type >= 2 ? (double)P : (double)tcount[p.target_panel];
+        const double ns = (p.interact_type & 1) ? (double)P : (double)scount[p.source_panel];
+        atomicAdd(&se[p.interact_type], nt * ns);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && se[threadIdx.x] != 0.0) atomicAdd(&evals[threadIdx.x], se[threadIdx.x]);
+}
+
+__global__ void k_group_gather(int L, const int *__restrict__ order, const btk_interact_pair *__restrict__ pairs,
+                               int *__restrict__ gsrc, unsigned char *__restrict__ gtype) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= L) return;
+    const btk_interact_pair p = pairs[order[i]];
+    gsrc[i] = p.source_panel;
+    gtype[i] = (unsigned char)p.interact_type;
+}
+
+__global__ void k_type_count(int L, const btk_interact_pair *__restrict__ pairs, unsigned long long *__restrict__ c) {
+    __shared__ unsigned int sc[4];
+    if (threadIdx.x < 4) sc[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i < L) atomicAdd(&sc[pairs[i].interact_type], 1u);
+    __syncthreads();
+    if (threadIdx.x < 4 && sc[threadIdx.x]) atomicAdd(&c[threadIdx.x], (unsigned long long)sc[threadIdx.x]);
+}
+
+}  // namespace
+
+std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree &ts, hipStream_t s) {
+    BTK_REQUIRE(tt.dim == ts.dim, "target and source trees differ in dimension");
+    const int dim = tt.dim;
+    const int NC = dim == 3 ? 8 : 4;
+    auto l = std::make_unique<List>();
+    l->dim = dim;
+    const TreeView T = view(tt), S = view(ts);
+    long long *hv = host_scratch().v;
+
+    DBuf<int2> cur(1), nxt;
+    const int2 root = make_int2(0, 0);
+    BTK_HIP(hipMemcpyAsync(cur.p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
+    int G = 1;
+    int L = 0;
+    size_t lcap = 1024;
+    l->pairs.resize(lcap);
+    while (G > 0) {
+        DBuf<unsigned char> code(G);
+        DBuf<long long> val(G), inc(G);
+        if (dim == 3)
+            k_classify<3><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, code.p,
+                                                 val.p);
+        else
+            k_classify<2><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, code.p,
+                                                 val.p);
+        BTK_LAUNCH_CHECK();
+        inclusive_sum_i64(val.p, inc.p, G, s);
+        BTK_HIP(hipMemcpyAsync(hv, inc.p + (G - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
+        BTK_HIP(hipStreamSynchronize(s));
+        const int E = (int)(hv[0] & 0xffffffffLL);
+        const long long X = hv[0] >> 32;
+        BTK_REQUIRE((long long)NC * X < (1ll << 31), "traversal generation exceeds 2^31 pairs");
+        if ((size_t)(L + E) > lcap) {
+            size_t ncap = lcap;
+            while (ncap < (size_t)(L + E)) ncap *= 2;
+            l->pairs.grow(ncap, s);
+            lcap = ncap;
+        }
+        const int Gn = (int)(NC * X);
+        nxt.resize(Gn ? Gn : 1);
+        if (dim == 3)
+            k_write<3><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
+        else
+            k_write<2><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
+        BTK_LAUNCH_CHECK();
+        L += E;
+        G = Gn;
+        std::swap(cur, nxt);
+    }
+    l->count = L;
+
+    // counters (interaction_list.cpp:242-249)
+    DBuf<unsigned long long> tc(4);
+    BTK_HIP(hipMemsetAsync(tc.p, 0, 4 * sizeof(unsigned long long), s));
+    if (L) {
+        k_type_count<<<nblk(L), TB, 0, s>>>(L, l->pairs.p, tc.p);
+        BTK_LAUNCH_CHECK();
+    }
+    BTK_HIP(hipMemcpyAsync(hv, tc.p, 4 * sizeof(long long), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 4; k++) l->counts[k] = hv[k];
+    cfg->fmm_interaction_count = L;
+    cfg->fmm_pp_count = (int)l->counts[0];
+    cfg->fmm_pc_count = (int)l->counts[1];
+    cfg->fmm_cp_count = (int)l->counts[2];
+    cfg->fmm_cc_count = (int)l->counts[3];
+
+    // target-grouped forms for the interaction kernels
+    const int pct = tt.pc;
+    int tbits = 1;
+    while ((1 << tbits) <= pct) tbits++;
+    const int P = ipow(cfg->interp_degree + 1, dim);
+    DBuf<int> keys(L ? L : 1), vals(L ? L : 1), keys2(L ? L : 1), order(L ? L : 1);
+    DBuf<int> cnt(2 * pct + 1);
+    DBuf<double> ev(4);
+    BTK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * pct + 1), s));
+    BTK_HIP(hipMemsetAsync(ev.p, 0, sizeof(double) * 4, s));
+    if (L) {
+        k_group_keys<<<nblk(L), TB, 0, s>>>(L, l->pairs.p, tbits, pct, keys.p, vals.p, cnt.p, tt.count.p,
+                                            ts.count.p, P, ev.p);
+        BTK_LAUNCH_CHECK();
+        sort_pairs_i32(keys.p, keys2.p, vals.p, order.p, L, tbits + 1, s);
+    }
+    l->goff.resize(2 * pct + 1);
+    exclusive_sum_i32(cnt.p, l->goff.p, 2 * pct + 1, s);
+    l->gsrc.resize(L ? L : 1);
+    l->gtype.resize(L ? L : 1);
+    if (L) {
+        k_group_gather<<<nblk(L), TB, 0, s>>>(L, order.p, l->pairs.p, l->gsrc.p, l->gtype.p);
+        BTK_LAUNCH_CHECK();
+    }
+    double evh[4];
+    BTK_HIP(hipMemcpyAsync(evh, ev.p, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 4; k++) l->evals[k] = evh[k];
+    l->pct = pct;
+    return l;
+}
+
+}  // namespace btk

@@ -1,0 +1,213 @@
+/*
+ * btk.h -- C ABI of the MI355X barycentric Lagrange dual-tree treecode.
+ *
+ * One entry point per entry point of the reference's hot path, with the
+ * reference's name prefixed by btk_.  Arguments are plain pointers and sizes:
+ * device pointers (d_*) address HBM on the current HIP device, host pointers
+ * are ordinary memory.  `stream` is a hipStream_t passed as void* (NULL is the
+ * legacy default stream).  Every function returns 0 on success and a nonzero
+ * status on failure; btk_last_error() then gives a thread-local message.  No
+ * exception crosses this boundary.
+ *
+ * Records that the reference exchanges by value (RunConfig, TreePanel_2d,
+ * TreePanel_3d, interact_pair, TreeInfo) have the reference's field order and
+ * x86-64 layout, so arrays copied out of the library compare byte for byte
+ * with the reference's own.
+ *
+ * Trees and interaction lists stay resident on the GPU behind opaque handles;
+ * btk_tree_copy_* and btk_list_copy bring their reference-layout form to the
+ * host on request.
+ */
+#ifndef BTK_H
+#define BTK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTK_MAX_DEGREE 10 /* barytreek-config.h.in:20 */
+
+/* One-byte boolean, as C++ bool in the reference's structs. */
+#ifdef __cplusplus
+#define BTK_BOOL bool
+#else
+#define BTK_BOOL _Bool
+#endif
+
+/* RunConfig (structs.hpp:6-21). */
+typedef struct btk_run_config {
+    int dim;
+    int interp_degree;
+    int interp_point_count;
+    double fmm_theta;
+    int fmm_cluster_thresh;
+    int fmm_interaction_count;
+    int fmm_pp_count;
+    int fmm_pc_count;
+    int fmm_cp_count;
+    int fmm_cc_count;
+    double ker_eps;
+    int mpi_p;
+    int mpi_id;
+} btk_run_config;
+
+/* TreeInfo (structs.hpp:23-27).  level_start is owned by the tree handle. */
+typedef struct btk_tree_info {
+    int panel_count;
+    int levels;
+    const int *level_start; /* [levels + 1] */
+} btk_tree_info;
+
+/* TreePanel_2d (structs.hpp:29-44), 80 bytes. */
+typedef struct btk_panel_2d {
+    int id, level, parent_id, child1, child2, child3, child4;
+    BTK_BOOL is_leaf;
+    double max_x, min_x, max_y, min_y, radius;
+    int point_count;
+} btk_panel_2d;
+
+/* TreePanel_3d (structs.hpp:46-67), 112 bytes. */
+typedef struct btk_panel_3d {
+    int id, level, parent_id, child1, child2, child3, child4, child5, child6, child7, child8;
+    BTK_BOOL is_leaf;
+    double max_x, min_x, max_y, min_y, min_z, max_z, radius;
+    int point_count;
+} btk_panel_3d;
+
+/* interact_pair (structs.hpp:74-78); type 0 = PP, 1 = PC, 2 = CP, 3 = CC. */
+typedef struct btk_interact_pair {
+    int target_panel;
+    int source_panel;
+    int interact_type;
+} btk_interact_pair;
+
+typedef struct btk_tree btk_tree; /* device-resident tree */
+typedef struct btk_list btk_list; /* device-resident interaction list */
+
+/* Interaction kernels.  Signs follow the reference unless convention = 1. */
+enum {
+    BTK_KERNEL_POT = 0,       /* 2D (1/4pi) log(r^2+eps^2); 3D -1/(4pi sqrt(r^2+eps^2)) (poisson_fmm_impl.hpp:38,241) */
+    BTK_KERNEL_REF_VEL3D = 1, /* 3D q (-1/4pi) (t-s)/(r^2+eps^2), 3 outputs (poisson_vel_fmm_impl.hpp:42,53) */
+    BTK_KERNEL_POT_GRAD = 2   /* potential and its target gradient, 1 + dim outputs (no reference counterpart) */
+};
+
+enum {
+    BTK_PREC_FP64 = 0,  /* every evaluation in FP64 */
+    BTK_PREC_MIXED = 1  /* FP32 kernel evaluations, FP64 geometry, weights and per-target sums */
+};
+
+/* Choices the reference makes implicitly.  Zero-initialised = reference behaviour. */
+typedef struct btk_options {
+    int kernel;       /* BTK_KERNEL_* */
+    int convention;   /* 0 = reference signs; 1 = physical signs (both kernels negated) */
+    int exclude_self; /* 1 = skip pairs at zero distance; 0 = include them (reference) */
+    int deepest_only; /* 1 = leaf_to_point on the deepest level only (downward_pass.cpp:26-28) */
+    int precision;    /* BTK_PREC_* */
+    int reserved[3];
+} btk_options;
+
+/* Number of output components of a kernel: 1, 3, or 1 + dim. */
+int btk_components(int dim, int kernel);
+
+/* Thread-local message of the last failure on this thread. */
+const char *btk_last_error(void);
+
+/* Releases the device memory the library keeps cached for reuse. */
+int btk_release_cached(void);
+
+/* read_run_config (read_namelist.hpp:7, read_namelist.cpp:10-44).
+ * cfg->dim must be set before the call, as in the reference drivers. */
+int btk_read_run_config(const char *path, btk_run_config *cfg);
+
+/* blfmm_tree_construction_2d/_3d (tree_construction.hpp:7-9).  Builds the
+ * tree on the GPU from n device-resident coordinates (original order). */
+int btk_tree_construction_2d(const btk_run_config *cfg, const double *d_x, const double *d_y, int n,
+                             void *stream, btk_tree **tree);
+int btk_tree_construction_3d(const btk_run_config *cfg, const double *d_x, const double *d_y,
+                             const double *d_z, int n, void *stream, btk_tree **tree);
+int btk_tree_get_info(const btk_tree *tree, btk_tree_info *info);
+/* blfmm_panels in the reference's AoS layout (host array of panel_count). */
+int btk_tree_copy_panels_2d(const btk_tree *tree, btk_panel_2d *panels);
+int btk_tree_copy_panels_3d(const btk_tree *tree, btk_panel_3d *panels);
+/* point_leaf_panel (host int[n]). */
+int btk_tree_copy_point_leaf_panel(const btk_tree *tree, int *point_leaf_panel);
+/* panel_points_inside, padded [panel_count][width], zero filled, leaf rows only
+ * (tree_construction.cpp:286-295); width is the reference's cluster_thresh. */
+int btk_tree_copy_panel_points_inside(const btk_tree *tree, int width, int *panel_points_inside);
+/* The same lists in compact form: perm[n] (point indices, leaf by leaf,
+ * ascending within a leaf) and start[panel_count + 1]; every panel, leaf or
+ * not, owns perm[start[p] .. start[p] + point_count). */
+int btk_tree_copy_permutation(const btk_tree *tree, int *perm, int *start);
+void btk_tree_destroy(btk_tree *tree);
+
+/* dual_tree_traversal_2d/_3d (interaction_list.hpp:6-8).  Writes the five
+ * counters into cfg as the reference does (interaction_list.cpp:242-249). */
+int btk_dual_tree_traversal_2d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
+                               btk_list **list);
+int btk_dual_tree_traversal_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
+                               btk_list **list);
+int btk_list_count(const btk_list *list);
+/* The full list in the reference's FIFO order (host array of count). */
+int btk_list_copy(const btk_list *list, btk_interact_pair *pairs);
+/* split_interactions (interaction_list.hpp:10): stable split by type. */
+int btk_split_interactions(const btk_list *list, btk_interact_pair *pp, btk_interact_pair *pc,
+                           btk_interact_pair *cp, btk_interact_pair *cc);
+void btk_list_destroy(btk_list *list);
+
+/* upward_pass_2d/_3d (upward_pass.hpp:7-9).  d_charges in original point
+ * order; writes d_proxy_source_weights[panel_count][P], P = (n+1)^dim, for
+ * every panel (no need to zero it first). */
+int btk_upward_pass_2d(const btk_run_config *cfg, const btk_tree *source, const double *d_charges,
+                       double *d_proxy_source_weights, void *stream);
+int btk_upward_pass_3d(const btk_run_config *cfg, const btk_tree *source, const double *d_charges,
+                       double *d_proxy_source_weights, void *stream);
+
+/* poisson_fmm_interactions_2d/_3d (poisson_fmm.hpp:9-15) and
+ * poisson_vel_fmm_interactions_3d (poisson_vel_fmm.hpp:9-11), the kernel
+ * chosen by opts.  Adds PP and PC to d_soln[C][M] (original target order)
+ * and CP and CC to d_proxy_target_weights[C][panel_count_target][P]. */
+int btk_fmm_interactions_2d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                            const btk_tree *source, const btk_list *list, const double *d_charges,
+                            const double *d_proxy_source_weights, double *d_soln,
+                            double *d_proxy_target_weights, void *stream);
+int btk_fmm_interactions_3d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                            const btk_tree *source, const btk_list *list, const double *d_charges,
+                            const double *d_proxy_source_weights, double *d_soln,
+                            double *d_proxy_target_weights, void *stream);
+
+/* downward_pass_2d/_3d/_3d_3 (downward_pass.hpp:7-11): parent_to_child
+ * through the levels, then leaf_to_point adding into d_soln[C][M]. */
+int btk_downward_pass_2d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                         double *d_proxy_target_weights, double *d_soln, void *stream);
+int btk_downward_pass_3d(const btk_run_config *cfg, const btk_options *opts, const btk_tree *target,
+                         double *d_proxy_target_weights, double *d_soln, void *stream);
+
+/* poisson_2d_dir_sum / poisson_3d_dir_sum / poisson_vel_3d_dir_sum
+ * (direct_sum_impl.hpp:9-94), FP64.  Adds into d_out[C][m]; target a is point
+ * d_target_idx[a] (or a when d_target_idx is NULL). */
+int btk_direct_sum_2d(const btk_options *opts, double eps, int m, const int *d_target_idx, const double *d_xt,
+                      const double *d_yt, int n, const double *d_xs, const double *d_ys, const double *d_q,
+                      double *d_out, void *stream);
+int btk_direct_sum_3d(const btk_options *opts, double eps, int m, const int *d_target_idx, const double *d_xt,
+                      const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                      const double *d_zs, const double *d_q, double *d_out, void *stream);
+
+/* Phase times (seconds, from HIP events) and counts of one btk_solve. */
+typedef struct btk_solve_stats {
+    double t_tree, t_traversal, t_upward, t_interact, t_downward, t_total;
+    long long count[4];  /* PP, PC, CP, CC pairs */
+    double evals[4];     /* kernel evaluations per type (sum n_t n_s, n_t P, P n_s, P^2) */
+    int panel_count_target, panel_count_source, levels_target, levels_source;
+} btk_solve_stats;
+
+/* The driver's sequence (example_poisson_3d.cpp:70-160) as one call: tree(s),
+ * traversal, upward, interactions, downward.  Adds into d_out[C][m].
+ * coincident != 0 means targets and sources are the same points (one tree). */
+int btk_solve(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
+              const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
+              const double *d_q, int coincident, double *d_out, btk_solve_stats *stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

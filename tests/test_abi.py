@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every symbol include/btk.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import barytreek_amd as btk
+from barytreek_amd import PANEL2D_DTYPE, PANEL3D_DTYPE, RunConfig, Options, SolveStats
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "btk.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"\b(btk_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_header_matches_binding_list():
+    assert declared_functions() == sorted(btk.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(btk.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_record_layouts_match_reference():
+    # structs.hpp:6-21 (RunConfig 64 B), :29-44 (TreePanel_2d 80 B), :46-67 (TreePanel_3d 112 B)
+    assert ctypes.sizeof(RunConfig) == 64
+    assert RunConfig.fmm_theta.offset == 16 and RunConfig.ker_eps.offset == 48
+    assert PANEL2D_DTYPE.itemsize == 80 and PANEL3D_DTYPE.itemsize == 112
+    assert ctypes.sizeof(Options) == 32
+    assert ctypes.sizeof(SolveStats) == 6 * 8 + 4 * 8 + 4 * 8 + 4 * 4
+
+
+def test_components():
+    assert btk.components(3, btk.KERNEL_POT) == 1
+    assert btk.components(3, btk.KERNEL_REF_VEL3D) == 3
+    assert btk.components(3, btk.KERNEL_POT_GRAD) == 4
+    assert btk.components(2, btk.KERNEL_POT_GRAD) == 3
