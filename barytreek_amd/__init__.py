@@ -62,7 +62,8 @@ class Options(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("t_tree", C.c_double), ("t_traversal", C.c_double), ("t_upward", C.c_double),
                 ("t_interact", C.c_double), ("t_downward", C.c_double), ("t_total", C.c_double),
-                ("count", C.c_longlong * 4), ("evals", C.c_double * 4), ("panel_count_target", C.c_int),
+                ("count", C.c_longlong * 4), ("evals", C.c_double * 4), ("t_kernel_near", C.c_double),
+                ("t_kernel_far", C.c_double), ("panel_count_target", C.c_int),
                 ("panel_count_source", C.c_int), ("levels_target", C.c_int), ("levels_source", C.c_int)]
 
 

@@ -459,9 +459,15 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
-        btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, d_out, Psi.p, s);
+        btk::KernelEvents kev;
+        for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
+        btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, d_out, Psi.p, s, &kev);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t4 = clk::now();
+        float ms_near = 0.f, ms_far = 0.f;
+        if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
+        if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
+        for (auto &e : kev.ev) (void)hipEventDestroy(e);
         btk::downward_pass(cfg, o, tt, Psi.p, d_out, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
@@ -476,6 +482,8 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
                 stats->count[k] = l->counts[k];
                 stats->evals[k] = l->evals[k];
             }
+            stats->t_kernel_near = 1e-3 * ms_near;
+            stats->t_kernel_far = 1e-3 * ms_far;
             stats->panel_count_target = tt.pc;
             stats->panel_count_source = ts->pc;
             stats->levels_target = tt.levels;

@@ -207,34 +207,46 @@ void build_tasks(const List &l, const Tree &tt, int far, int P, int per, Tasks &
 
 template <int DIM, int KERNEL>
 void launch_interactions(const btk_run_config &cfg, const KParams &kp, const Tree &tt, const Tree &ts, const List &l,
-                         const SrcView &sv, const NodeTables &tabs, double *out, double *Psi, hipStream_t s) {
+                         const SrcView &sv, const NodeTables &tabs, double *out, double *Psi, hipStream_t s,
+                         KernelEvents *kev) {
     const int n1 = cfg.interp_degree + 1;
     const int P = ipow(n1, DIM);
     if (l.counts[0] + l.counts[1] > 0) {
         Tasks tk;
         build_tasks(l, tt, 0, P, 256, tk, s);
+        if (kev) BTK_HIP(hipEventRecord(kev->ev[0], s));
         if (tk.n)
             k_near<DIM, KERNEL, 256><<<tk.n, 256, 0, s>>>(tk.panel.p, tk.first.p, l.goff.p, l.gsrc.p, l.gtype.p,
                                                          tt.start.p, tt.count.p, tt.sx.p, tt.sy.p, tt.sz.p, tt.perm.p,
                                                          sv, n1, tabs, kp, out, tt.n);
         BTK_LAUNCH_CHECK();
+        if (kev) {
+            BTK_HIP(hipEventRecord(kev->ev[1], s));
+            kev->near_done = true;
+        }
     }
     if (l.counts[2] + l.counts[3] > 0) {
         Tasks tk;
         if (P <= 64) {
             build_tasks(l, tt, 1, P, 64, tk, s);
+            if (kev) BTK_HIP(hipEventRecord(kev->ev[2], s));
             if (tk.n)
                 k_far<DIM, KERNEL, 64><<<tk.n, 64, 0, s>>>(tk.panel.p, tk.first.p, l.goff.p, l.gsrc.p, l.gtype.p,
                                                           tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p,
                                                           tt.hi[2].p, sv, n1, tabs, kp, Psi, tt.pc);
         } else {
             build_tasks(l, tt, 1, P, 256, tk, s);
+            if (kev) BTK_HIP(hipEventRecord(kev->ev[2], s));
             if (tk.n)
                 k_far<DIM, KERNEL, 256><<<tk.n, 256, 0, s>>>(tk.panel.p, tk.first.p, l.goff.p, l.gsrc.p, l.gtype.p,
                                                             tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p,
                                                             tt.hi[1].p, tt.hi[2].p, sv, n1, tabs, kp, Psi, tt.pc);
         }
         BTK_LAUNCH_CHECK();
+        if (kev) {
+            BTK_HIP(hipEventRecord(kev->ev[3], s));
+            kev->far_done = true;
+        }
     }
 }
 
@@ -277,7 +289,8 @@ __global__ void __launch_bounds__(TB)
 }  // namespace
 
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts, const List &l,
-                      const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s) {
+                      const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s,
+                      KernelEvents *kev) {
     BTK_REQUIRE(tt.dim == ts.dim, "target and source trees differ in dimension");
     const int n1 = cfg.interp_degree + 1;
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
@@ -287,17 +300,17 @@ void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tre
                      ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W};
     if (tt.dim == 3) {
         if (o.kernel == BTK_KERNEL_POT)
-            launch_interactions<3, BTK_KERNEL_POT>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s);
+            launch_interactions<3, BTK_KERNEL_POT>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
         else if (o.kernel == BTK_KERNEL_REF_VEL3D)
-            launch_interactions<3, BTK_KERNEL_REF_VEL3D>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s);
+            launch_interactions<3, BTK_KERNEL_REF_VEL3D>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
         else
-            launch_interactions<3, BTK_KERNEL_POT_GRAD>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s);
+            launch_interactions<3, BTK_KERNEL_POT_GRAD>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
     } else {
         BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "the reference vel kernel is 3D only");
         if (o.kernel == BTK_KERNEL_POT)
-            launch_interactions<2, BTK_KERNEL_POT>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s);
+            launch_interactions<2, BTK_KERNEL_POT>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
         else
-            launch_interactions<2, BTK_KERNEL_POT_GRAD>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s);
+            launch_interactions<2, BTK_KERNEL_POT_GRAD>(cfg, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
     }
 }
 

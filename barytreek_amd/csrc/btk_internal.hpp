@@ -170,9 +170,15 @@ std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const dou
 std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &t, const Tree &s, hipStream_t st);
 void upward_pass(const btk_run_config &cfg, const Tree &src, const double *q_sorted, double *W, hipStream_t s);
 void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_t s);
+// Optional HIP events recorded around the near (PP+PC) and far (CP+CC)
+// kernel launches: ev[0], ev[1] bracket the near kernel, ev[2], ev[3] the far.
+struct KernelEvents {
+    hipEvent_t ev[4];
+    bool near_done = false, far_done = false;
+};
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
                       const List &l, const double *q_sorted, const double *W, double *out, double *Psi,
-                      hipStream_t s);
+                      hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
                    hipStream_t s);
 void direct_sum(int dim, const btk_options &o, double eps, int m, const int *idx, const double *xt,

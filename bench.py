@@ -1,0 +1,165 @@
+"""Benchmark of the MI355X treecode on the 2D Poisson potential+gradient workload.
+
+Workload (one step = one full solve: tree, traversal, upward, interactions,
+downward, on inputs already resident in HBM):
+    N = M = 1,000,000 coincident points in 2D: 800k from 4 isotropic Gaussians
+    (sigma 0.05, centres U[-1,1]^2) + 200k uniform in [-1,1]^2; charges N(0,1);
+    kernel -log r / (2 pi) and its gradient; interpolation degree 6, theta 0.6,
+    leaf size 1000; eps = 0 with zero-distance (self) pairs excluded.
+
+Prints one JSON line.  `value` is targets/s of the whole job.  The accuracy of
+every variant is its relative L2 error over all targets (potential and both
+gradient components) against an FP64 direct sum on the GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant plain|mixed]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N_POINTS = 1_000_000
+DEGREE, THETA, LEAF = 6, 0.6, 1000
+SEED = 20261203
+FLOPS_PER_EVAL = 15  # 2D potential + gradient (FMA = 2, transcendental = 1)
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variant", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--n", type=int, default=N_POINTS)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
+    return ap.parse_args()
+
+
+def cpu_baseline(x, y, q, stride):
+    """The CPU restatement (oracle) of the same workload, OpenMP on the host's
+    cores: tree, traversal, upward pass, CP/CC and downward pass over all
+    targets, PP/PC on every `stride`-th pair, scaled by the sampled fraction of
+    PP/PC evaluations."""
+    from oracle import pyoracle as O
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))
+    O.set_threads(cores)
+    cfg = O.make_config(2, DEGREE, LEAF, THETA, 0.0)
+    o = O.opts(kernel=O.KERNEL_POT_GRAD, convention=1, exclude_self=1, near_stride=stride)
+    t0 = time.perf_counter()
+    _, t, _ = O.solve(cfg, o, x, y, None, x, y, None, q)
+    wall = time.perf_counter() - t0
+    near_s, frac = O.last_near()
+    est = wall - near_s + near_s / max(frac, 1e-12)
+    return {
+        "value": len(x) / est,
+        "unit": "targets/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle (C/OpenMP restatement) on the same 1M-point problem: tree, traversal, upward, "
+                   f"CP/CC and downward over all targets; PP/PC on every {stride}th pair "
+                   f"({frac:.3f} of PP/PC evaluations, {near_s:.1f} s), scaled; measured {wall:.1f} s, "
+                   f"estimated full solve {est:.1f} s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by target "
+              f"subtrees, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import barytreek_amd as btk
+    from barytreek_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    x, y, _, q = wl.gaussian_mixture_2d(args.n, SEED)
+    dx, dy, dq = (torch.from_numpy(a).to(dev) for a in (x, y, q))
+    cfg = btk.make_config(2, DEGREE, LEAF, THETA, 0.0)
+    prec = btk.PREC_FP64 if args.variant == "plain" else btk.PREC_MIXED
+    opts = btk.options(kernel=btk.KERNEL_POT_GRAD, convention=1, exclude_self=1, precision=prec)
+    C = btk.components(2, btk.KERNEL_POT_GRAD)
+    out = torch.zeros(C * args.n, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        out.zero_()
+        btk.solve(cfg, opts, dx, dy, None, dq, out, stream=stream)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out.zero_()
+        stats.append(btk.solve(cfg, opts, dx, dy, None, dq, out, stream=stream))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) / args.steps * 1e3
+    value = args.n / (ms * 1e-3)
+
+    def avg(f):
+        return float(np.mean([getattr(s, f) for s in stats]))
+
+    evals = [float(stats[-1].evals[k]) for k in range(4)]
+    t_near = avg("t_kernel_near")
+    near_flops = (evals[0] + evals[1]) * FLOPS_PER_EVAL
+    dtype = "f64" if args.variant == "plain" else "f32"
+    achieved = near_flops / t_near / 1e12 if t_near > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_TFLOPS[dtype], 4), "traffic": None,
+                "kernel": "k_near (PP+PC)", "flops_per_launch": near_flops,
+                "avg_launch_s": t_near}
+
+    rel_l2 = None
+    if not args.no_accuracy:
+        ds = torch.zeros_like(out)
+        btk.direct_sum(2, opts, 0.0, dx, dy, None, dx, dy, None, dq, ds)
+        torch.cuda.synchronize()
+        rel_l2 = float(torch.linalg.norm(out - ds) / torch.linalg.norm(ds))
+        rel_l2_pot = float(torch.linalg.norm(out[: args.n] - ds[: args.n]) / torch.linalg.norm(ds[: args.n]))
+
+    line = {
+        "metric": "targets/s at <=1e-6 rel-L2 vs FP64 direct sum; plain vs mixed/batched/learned",
+        "value": value,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": "2D Poisson -log r/(2pi) potential+gradient, N=M=1,000,000 (4 Gaussians sigma 0.05 "
+                               "+ 20% uniform), q~N(0,1), degree 6, theta 0.6, leaf 1000, eps 0, self excluded",
+                   "variant": args.variant, "n": args.n, "seed": SEED, "parallelism": "single GPU"},
+        "rel_l2": rel_l2,
+        "rel_l2_potential": rel_l2_pot if rel_l2 is not None else None,
+        "interactions_per_s": sum(evals) / avg("t_interact"),
+        "evals": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
+        "pairs": [int(stats[-1].count[k]) for k in range(4)],
+        "phases_s": {k: avg("t_" + k) for k in ("tree", "traversal", "upward", "interact", "downward", "total")},
+        "t_kernel_far_s": avg("t_kernel_far"),
+        "panels": int(stats[-1].panel_count_source), "levels": int(stats[-1].levels_source),
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(x, y, q, args.cpu_stride)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

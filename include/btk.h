@@ -197,6 +197,8 @@ typedef struct btk_solve_stats {
     double t_tree, t_traversal, t_upward, t_interact, t_downward, t_total;
     long long count[4];  /* PP, PC, CP, CC pairs */
     double evals[4];     /* kernel evaluations per type (sum n_t n_s, n_t P, P n_s, P^2) */
+    double t_kernel_near; /* device time of the PP+PC kernel (HIP events on the solve's stream) */
+    double t_kernel_far;  /* device time of the CP+CC kernel */
     int panel_count_target, panel_count_source, levels_target, levels_source;
 } btk_solve_stats;
 

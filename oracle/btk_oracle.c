@@ -696,6 +696,11 @@ static inline void proxy_point(int dim, int n1, int idx, const double *cx, const
     }
 }
 
+static double g_near_seconds = 0, g_near_fraction = 1;
+double orc_last_near_seconds(void) { return g_near_seconds; }
+double orc_last_near_fraction(void) { return g_near_fraction; }
+static double now_s(void);
+
 void orc_fmm_interactions(const orc_run_config *cfg, const orc_opts *opt, const orc_tree *tt, const orc_tree *ts,
                           const double *xt, const double *yt, const double *zt, const double *xs,
                           const double *ys, const double *zs, const double *q, const double *W,
@@ -707,12 +712,28 @@ void orc_fmm_interactions(const orc_run_config *cfg, const orc_opts *opt, const 
     const size_t pcT = (size_t)tt->panel_count;
     const kparams kp = make_kparams(dim, cfg->ker_eps, opt);
     static const double zero3[1] = {0};
+    const int stride = opt->near_stride > 1 ? opt->near_stride : 1;
+    double near_all = 0, near_done = 0;
+    for (int i = 0; i < count; i++) {
+        int ty = list[i].interact_type;
+        if (ty > 1) continue;
+        int it = list[i].target_panel, is = list[i].source_panel;
+        double e = (double)(tt->leaf_start[it + 1] - tt->leaf_start[it]) *
+                   (ty == 0 ? (double)(ts->leaf_start[is + 1] - ts->leaf_start[is]) : (double)P);
+        near_all += e;
+        if (i % stride == 0) near_done += e;
+    }
+    g_near_fraction = near_all > 0 ? near_done / near_all : 1.0;
+    double tn0 = 0;
 
     /* one launch per type, PP, PC, CP, CC (poisson_fmm.cpp:21-24) */
     for (int type = 0; type < 4; type++) {
+        if (type == 0) tn0 = now_s();
+        if (type == 2) g_near_seconds = now_s() - tn0;
 #pragma omp parallel for schedule(dynamic)
         for (int i = 0; i < count; i++) {
             if (list[i].interact_type != type) continue;
+            if (type < 2 && i % stride != 0) continue;
             const int it = list[i].target_panel, is = list[i].source_panel;
             const orc_panel *T = &tt->panels[it], *S = &ts->panels[is];
             double tcx[ORC_MAX_DEGREE + 1], tcy[ORC_MAX_DEGREE + 1], tcz[ORC_MAX_DEGREE + 1];

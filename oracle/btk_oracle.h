@@ -105,7 +105,13 @@ typedef struct orc_opts {
     int convention;      /* 0 = reference signs (poisson_fmm_impl.hpp:38,241); 1 = physical (negated) */
     int exclude_self;    /* 1 = skip pairs at zero distance (dx=dy=dz=0) */
     int deepest_only;    /* 1 = leaf_to_point only on the deepest level (downward_pass.cpp:14-16) */
+    int near_stride;     /* >1: evaluate only every near_stride-th PP/PC pair (CPU-baseline sampling) */
 } orc_opts;
+
+/* Seconds spent in the PP/PC loops of the last orc_fmm_interactions call, and
+ * the fraction of PP/PC kernel evaluations that call performed. */
+double orc_last_near_seconds(void);
+double orc_last_near_fraction(void);
 
 int orc_components(int dim, int kernel);
 

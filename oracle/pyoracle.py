@@ -52,7 +52,7 @@ class Pair(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
-                ("deepest_only", C.c_int)]
+                ("deepest_only", C.c_int), ("near_stride", C.c_int)]
 
 
 PAIR_DTYPE = np.dtype([("target_panel", "<i4"), ("source_panel", "<i4"), ("interact_type", "<i4")])
@@ -112,6 +112,8 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         L.orc_components.argtypes = [C.c_int, C.c_int]
+        L.orc_last_near_seconds.restype = C.c_double
+        L.orc_last_near_fraction.restype = C.c_double
         L.free = C.CDLL(None).free
         L.free.argtypes = [C.c_void_p]
         _lib = L
@@ -229,10 +231,16 @@ def split_interactions(lst: np.ndarray, cfg: RunConfig):
     return pp, pc, cp, cc
 
 
-def opts(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0) -> Opts:
+def opts(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, near_stride=1) -> Opts:
     o = Opts()
     o.kernel, o.convention, o.exclude_self, o.deepest_only = kernel, convention, exclude_self, deepest_only
+    o.near_stride = near_stride
     return o
+
+
+def last_near():
+    """(seconds in the PP/PC loops, fraction of PP/PC evaluations performed) of the last call."""
+    return lib().orc_last_near_seconds(), lib().orc_last_near_fraction()
 
 
 def components(dim, kernel):
