@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbtk.so")
+DRIVER = os.path.join(HERE, "btk_driver")
 REPO = os.path.dirname(HERE)
 
 SOURCES = [
@@ -67,8 +68,16 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # namelist-driven driver over the C++ drop-in layer (cpp/barytreek.hpp)
+    cmd = [hipcc(), "-O2", "-std=c++20", "-I", os.path.join(REPO, "include"), "-I", os.path.join(HERE, "cpp"),
+           os.path.join(HERE, "cpp", "btk_driver.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-lbtk",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

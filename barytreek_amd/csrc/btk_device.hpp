@@ -138,4 +138,57 @@ __device__ __forceinline__ double comp_scale(int c, double gfc) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 evaluations (mixed precision).  Coordinates are local to a frame
+// centre chosen per target block (FP64 difference, then rounded), so the
+// FP32 differences carry the relative precision of the local extent, not of
+// the absolute position.  The 2D potential accumulates q log2 R (the
+// hardware logarithm), rescaled by ln 2 in comp_scale_f32.
+// ---------------------------------------------------------------------------
+template <int DIM, int KERNEL>
+__device__ __forceinline__ void eval_f32(float tx, float ty, float tz, float4 s, float eps2, int excl, float *acc) {
+    const float dx = tx - s.x, dy = ty - s.y;
+    if constexpr (DIM == 3) {
+        const float dz = tz - s.z;
+        const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const bool skip = excl && dx == 0.0f && dy == 0.0f && dz == 0.0f;
+        if constexpr (KERNEL == BTK_KERNEL_POT) {
+            const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
+            acc[0] = fmaf(s.w, inv, acc[0]);
+        } else if constexpr (KERNEL == BTK_KERNEL_REF_VEL3D) {
+            const float g = skip ? 0.0f : s.w * __builtin_amdgcn_rcpf(R);
+            acc[0] = fmaf(dx, g, acc[0]);
+            acc[1] = fmaf(dy, g, acc[1]);
+            acc[2] = fmaf(dz, g, acc[2]);
+        } else {
+            const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
+            const float qi = s.w * inv;
+            acc[0] += qi;
+            const float h = qi * inv * inv;
+            acc[1] = fmaf(h, dx, acc[1]);
+            acc[2] = fmaf(h, dy, acc[2]);
+            acc[3] = fmaf(h, dz, acc[3]);
+        }
+    } else {
+        (void)tz;
+        const float R = fmaf(dx, dx, fmaf(dy, dy, eps2));
+        const bool skip = excl && dx == 0.0f && dy == 0.0f;
+        const float lg = skip ? 0.0f : __builtin_amdgcn_logf(R);
+        acc[0] = fmaf(s.w, lg, acc[0]);
+        if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
+            const float g = skip ? 0.0f : s.w * __builtin_amdgcn_rcpf(R);
+            acc[1] = fmaf(g, dx, acc[1]);
+            acc[2] = fmaf(g, dy, acc[2]);
+        }
+    }
+}
+
+template <int DIM, int KERNEL>
+__device__ __forceinline__ double comp_scale_f32(int c, double gfc) {
+    if constexpr (DIM == 2) {
+        if (c == 0) return gfc * 0.69314718055994530942;  // log R = ln 2 * log2 R
+    }
+    return comp_scale<DIM, KERNEL>(c, gfc);
+}
+
 }  // namespace btk

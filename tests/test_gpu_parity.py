@@ -175,3 +175,38 @@ def test_solve_matches_oracle_and_direct_sum(kind, dim, n, deg, thresh, theta, e
     assert e < TOL
     assert e_ds < TOL
     assert l2 < 1e-4
+
+
+MIXED_CASES = [
+    ("gauss2", 2, 60000, 6, 500, 0.6, 0.0, btk.KERNEL_POT_GRAD, 1),
+    ("cube2", 2, 30000, 6, 300, 0.6, 1e-4, btk.KERNEL_POT, 0),
+    ("cube3", 3, 40000, 8, 500, 0.7, 0.0, btk.KERNEL_POT_GRAD, 1),
+    ("sphere3", 3, 30000, 6, 300, 0.7, 1e-4, btk.KERNEL_POT, 0),
+    ("cube3", 3, 20000, 6, 300, 0.7, 1e-4, btk.KERNEL_REF_VEL3D, 0),
+]
+
+MIXED_TOL = 1e-6  # FP32 evaluation, FP64 accumulation: rel-L2 distance to the FP64 path
+
+
+@pytest.mark.parametrize("kind,dim,n,deg,thresh,theta,eps,kernel,excl", MIXED_CASES)
+def test_mixed_precision_close_to_fp64(kind, dim, n, deg, thresh, theta, eps, kernel, excl):
+    print(f"\n[mixed] {kind} dim={dim} n={n} deg={deg} kernel={kernel}", flush=True)
+    x, y, z, q = points(kind, n, 300 + n)
+    cfg = btk.make_config(dim, deg, thresh, theta, eps)
+    Cn = btk.components(dim, kernel)
+    d = [dev(a) for a in (x, y, z, q)]
+    res = {}
+    for prec in (btk.PREC_FP64, btk.PREC_MIXED):
+        opts = btk.options(kernel=kernel, exclude_self=excl, precision=prec)
+        out = torch.zeros(Cn * n, dtype=torch.float64, device="cuda")
+        btk.solve(cfg, opts, d[0], d[1], d[2], d[3], out)
+        res[prec] = out.cpu().numpy()
+    ds = torch.zeros(Cn * n, dtype=torch.float64, device="cuda")
+    btk.direct_sum(dim, btk.options(kernel=kernel, exclude_self=excl), eps, d[0], d[1], d[2], d[0], d[1], d[2],
+                   d[3], ds)
+    dsn = ds.cpu().numpy()
+    e = rel(res[btk.PREC_MIXED], res[btk.PREC_FP64])
+    l64, l32 = rel(res[btk.PREC_FP64], dsn), rel(res[btk.PREC_MIXED], dsn)
+    print(f"  mixed vs fp64 {e:.2e}; rel-L2 vs direct sum: fp64 {l64:.2e} mixed {l32:.2e}", flush=True)
+    assert e < MIXED_TOL
+    assert l32 < l64 + MIXED_TOL
