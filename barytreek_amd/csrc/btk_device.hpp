@@ -139,30 +139,29 @@ __device__ __forceinline__ double comp_scale(int c, double gfc) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 evaluations (mixed precision).  Coordinates are local to a frame
-// centre chosen per target block (FP64 difference, then rounded), so the
-// FP32 differences carry the relative precision of the local extent, not of
-// the absolute position.  The 2D potential accumulates q log2 R (the
-// hardware logarithm), rescaled by ln 2 in comp_scale_f32.
+// FP32 evaluations (mixed precision).  The difference d = t - s is formed in
+// FP64 and rounded once, so it keeps FP32 relative precision even for pairs
+// much closer than their coordinates' magnitude; R = |d|^2 + eps^2, the
+// hardware log2 / reciprocal / reciprocal square root and the products are
+// FP32.  The 2D potential accumulates q log2 R, rescaled by ln 2 in
+// comp_scale_f32.
 // ---------------------------------------------------------------------------
 template <int DIM, int KERNEL>
-__device__ __forceinline__ void eval_f32(float tx, float ty, float tz, float4 s, float eps2, int excl, float *acc) {
-    const float dx = tx - s.x, dy = ty - s.y;
+__device__ __forceinline__ void eval_f32(float dx, float dy, float dz, float q, float eps2, int excl, float *acc) {
     if constexpr (DIM == 3) {
-        const float dz = tz - s.z;
         const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         const bool skip = excl && dx == 0.0f && dy == 0.0f && dz == 0.0f;
         if constexpr (KERNEL == BTK_KERNEL_POT) {
             const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
-            acc[0] = fmaf(s.w, inv, acc[0]);
+            acc[0] = fmaf(q, inv, acc[0]);
         } else if constexpr (KERNEL == BTK_KERNEL_REF_VEL3D) {
-            const float g = skip ? 0.0f : s.w * __builtin_amdgcn_rcpf(R);
+            const float g = skip ? 0.0f : q * __builtin_amdgcn_rcpf(R);
             acc[0] = fmaf(dx, g, acc[0]);
             acc[1] = fmaf(dy, g, acc[1]);
             acc[2] = fmaf(dz, g, acc[2]);
         } else {
             const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
-            const float qi = s.w * inv;
+            const float qi = q * inv;
             acc[0] += qi;
             const float h = qi * inv * inv;
             acc[1] = fmaf(h, dx, acc[1]);
@@ -170,13 +169,13 @@ __device__ __forceinline__ void eval_f32(float tx, float ty, float tz, float4 s,
             acc[3] = fmaf(h, dz, acc[3]);
         }
     } else {
-        (void)tz;
+        (void)dz;
         const float R = fmaf(dx, dx, fmaf(dy, dy, eps2));
         const bool skip = excl && dx == 0.0f && dy == 0.0f;
         const float lg = skip ? 0.0f : __builtin_amdgcn_logf(R);
-        acc[0] = fmaf(s.w, lg, acc[0]);
+        acc[0] = fmaf(q, lg, acc[0]);
         if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
-            const float g = skip ? 0.0f : s.w * __builtin_amdgcn_rcpf(R);
+            const float g = skip ? 0.0f : q * __builtin_amdgcn_rcpf(R);
             acc[1] = fmaf(g, dx, acc[1]);
             acc[2] = fmaf(g, dy, acc[2]);
         }

@@ -19,11 +19,10 @@
 // by every lane as LDS broadcasts.  Results are written once per target.
 //
 // Precision.  The FP64 path evaluates (t - s) on absolute coordinates as the
-// reference does.  The mixed path (BTK_PREC_MIXED) evaluates in FP32 in a
-// frame centred on the target panel (each coordinate shifted in FP64, then
-// rounded), with hardware log2 / reciprocal / reciprocal square root; FP32
-// partial sums run over at most one 64-source tile and are then added into
-// FP64 per-target accumulators.
+// reference does.  The mixed path (BTK_PREC_MIXED) forms t - s in FP64,
+// rounds it once to FP32 and evaluates the kernel in FP32 with the hardware
+// log2 / reciprocal / reciprocal square root; FP32 partial sums run over at
+// most one source tile and are then added into FP64 per-target accumulators.
 #include "btk_device.hpp"
 
 namespace btk {
@@ -45,19 +44,11 @@ struct Src64 {
 };
 
 template <bool F32>
-using TileT = typename std::conditional<F32, float4, Src64>::type;
+using TileT = Src64;  // FP64 coordinates in both paths: differences are formed in FP64
 
 template <bool F32>
-__device__ __forceinline__ TileT<F32> make_src(double x, double y, double z, double q, double cx, double cy,
-                                               double cz) {
-    if constexpr (F32) {
-        return make_float4((float)(x - cx), (float)(y - cy), (float)(z - cz), (float)q);
-    } else {
-        (void)cx;
-        (void)cy;
-        (void)cz;
-        return Src64{x, y, z, q};
-    }
+__device__ __forceinline__ Src64 make_src(double x, double y, double z, double q, double, double, double) {
+    return Src64{x, y, z, q};
 }
 
 template <int DIM>
@@ -96,7 +87,9 @@ template <int DIM, int KERNEL, bool F32>
 __device__ __forceinline__ void eval_any(const TileT<F32> &s, const double *t64, const float *t32, double eps2,
                                          int excl, double *acc64, float *acc32) {
     if constexpr (F32) {
-        eval_f32<DIM, KERNEL>(t32[0], t32[1], t32[2], s, (float)eps2, excl, acc32);
+        (void)t32;
+        eval_f32<DIM, KERNEL>((float)(t64[0] - s.x), (float)(t64[1] - s.y), DIM == 3 ? (float)(t64[2] - s.z) : 0.0f,
+                              (float)s.q, (float)eps2, excl, acc32);
     } else {
         eval_f64<DIM, KERNEL>(t64[0], t64[1], t64[2], s.x, s.y, s.z, s.q, eps2, excl, acc64);
     }
