@@ -25,6 +25,11 @@ SOURCES = [
     "btk_traversal.hip",
     "btk_passes.hip",
     "btk_interact.hip",
+    "btk_ik_2d_pot.hip",
+    "btk_ik_2d_grad.hip",
+    "btk_ik_3d_pot.hip",
+    "btk_ik_3d_vel.hip",
+    "btk_ik_3d_grad.hip",
     "btk_api.hip",
 ]
 

@@ -78,22 +78,20 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 //   POT        acc0 = sum q G0          (3D G0 = 1/R^(1/2), 2D G0 = log R)
 //   REF_VEL3D  acc  = sum q d / R       (3 components)
 //   POT_GRAD   acc0 as POT; acc1.. = sum q d / R^(3/2) (3D), sum q d / R (2D)
-// with R = r^2 + eps^2 and d = t - s.  A pair is skipped when excl is set
-// and d = 0.
+// with R = r^2 + eps^2 and d = t - s.
 // ---------------------------------------------------------------------------
 template <int DIM, int KERNEL>
 struct KernelTraits {
     static constexpr int C = (KERNEL == BTK_KERNEL_POT) ? 1 : (KERNEL == BTK_KERNEL_REF_VEL3D ? 3 : 1 + DIM);
 };
 
-template <int DIM, int KERNEL>
-__device__ __forceinline__ void eval_f64(double tx, double ty, double tz, double sx, double sy, double sz, double q,
-                                         double eps2, int excl, double *acc) {
-    const double dx = tx - sx, dy = ty - sy;
+// d = t - s is passed in; EXCL skips the pair when R == eps^2, i.e. when
+// |d|^2 vanishes at working precision (the self pair of coincident sets).
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double q, double eps2, double *acc) {
     if constexpr (DIM == 3) {
-        const double dz = tz - sz;
         const double R = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-        const bool skip = excl && dx == 0.0 && dy == 0.0 && dz == 0.0;
+        const bool skip = EXCL && R == eps2;
         if constexpr (KERNEL == BTK_KERNEL_POT) {
             const double inv = skip ? 0.0 : rsqrt_f64(R);
             acc[0] = fma(q, inv, acc[0]);
@@ -112,10 +110,9 @@ __device__ __forceinline__ void eval_f64(double tx, double ty, double tz, double
             acc[3] = fma(h, dz, acc[3]);
         }
     } else {
-        (void)tz;
-        (void)sz;
+        (void)dz;
         const double R = fma(dx, dx, fma(dy, dy, eps2));
-        const bool skip = excl && dx == 0.0 && dy == 0.0;
+        const bool skip = EXCL && R == eps2;
         const double lg = skip ? 0.0 : log(R);
         acc[0] = fma(q, lg, acc[0]);
         if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
@@ -146,11 +143,11 @@ __device__ __forceinline__ double comp_scale(int c, double gfc) {
 // FP32.  The 2D potential accumulates q log2 R, rescaled by ln 2 in
 // comp_scale_f32.
 // ---------------------------------------------------------------------------
-template <int DIM, int KERNEL>
-__device__ __forceinline__ void eval_f32(float dx, float dy, float dz, float q, float eps2, int excl, float *acc) {
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_f32(float dx, float dy, float dz, float q, float eps2, float *acc) {
     if constexpr (DIM == 3) {
         const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-        const bool skip = excl && dx == 0.0f && dy == 0.0f && dz == 0.0f;
+        const bool skip = EXCL && R == eps2;
         if constexpr (KERNEL == BTK_KERNEL_POT) {
             const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
             acc[0] = fmaf(q, inv, acc[0]);
@@ -171,7 +168,7 @@ __device__ __forceinline__ void eval_f32(float dx, float dy, float dz, float q, 
     } else {
         (void)dz;
         const float R = fmaf(dx, dx, fmaf(dy, dy, eps2));
-        const bool skip = excl && dx == 0.0f && dy == 0.0f;
+        const bool skip = EXCL && R == eps2;
         const float lg = skip ? 0.0f : __builtin_amdgcn_logf(R);
         acc[0] = fmaf(q, lg, acc[0]);
         if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {

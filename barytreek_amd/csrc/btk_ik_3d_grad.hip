@@ -1,0 +1,6 @@
+// Interaction kernels, 3D BTK_KERNEL_POT_GRAD (one translation unit per kernel keeps builds parallel).
+#include "btk_interact.hpp"
+
+namespace btk {
+template void launch_interact<3, BTK_KERNEL_POT_GRAD>(const InteractLaunch &, hipStream_t);
+}  // namespace btk

@@ -11,7 +11,8 @@ Prints one JSON line.  `value` is targets/s of the whole job.  The accuracy of
 every variant is its relative L2 error over all targets (potential and both
 gradient components) against an FP64 direct sum on the GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant plain|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant both|plain|mixed]
+                    [--workload 2d|3d-uniform-100k|3d-sphere-plummer-4M]
 """
 from __future__ import annotations
 
@@ -26,11 +27,23 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-N_POINTS = 1_000_000
-DEGREE, THETA, LEAF = 6, 0.6, 1000
 SEED = 20261203
-FLOPS_PER_EVAL = 15  # 2D potential + gradient (FMA = 2, transcendental = 1)
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+# flops per kernel evaluation (FMA = 2, transcendental = 1)
+FLOPS = {(2, 0): 9, (2, 2): 15, (3, 0): 12, (3, 1): 17, (3, 2): 20}
+
+# name: (dim, n, generator, degree, theta, leaf, kernel, description)
+WORKLOADS = {
+    "2d": (2, 1_000_000, "gauss2", 6, 0.6, 1000, 2,
+           "2D Poisson -log r/(2pi) potential+gradient, N=M=1,000,000 (4 Gaussians sigma 0.05 + 20% uniform), "
+           "q~N(0,1), degree 6, theta 0.6, leaf 1000, eps 0, self excluded"),
+    "3d-uniform-100k": (3, 100_000, "cube3", 8, 0.7, 500, 0,
+                        "3D Poisson 1/(4pi r) potential, N=M=100,000 uniform in [-1,1]^3, q~U[-1,1], degree 8, "
+                        "theta 0.7, leaf 500, eps 0, self excluded"),
+    "3d-sphere-plummer-4M": (3, 4_000_000, "sphere3", 8, 0.7, 2000, 2,
+                             "3D potential+gradient, N=M=4,000,000 (half unit sphere, half Plummer a=0.3 r<=2), "
+                             "q~U[-1,1], degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
+}
 
 
 def parse():
@@ -40,14 +53,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", choices=["both", "plain", "mixed"], default="both")
     ap.add_argument("--no-degree-scan", action="store_true")
-    ap.add_argument("--n", type=int, default=N_POINTS)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d")
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
     return ap.parse_args()
 
 
-def cpu_baseline(x, y, q, stride):
+def cpu_baseline(wl_spec, x, y, z, q, stride):
     """The CPU restatement (oracle) of the same workload, OpenMP on the host's
     cores: tree, traversal, upward pass, CP/CC and downward pass over all
     targets, PP/PC on every `stride`-th pair, scaled by the sampled fraction of
@@ -56,10 +70,11 @@ def cpu_baseline(x, y, q, stride):
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(cores, 16))
     O.set_threads(cores)
-    cfg = O.make_config(2, DEGREE, LEAF, THETA, 0.0)
-    o = O.opts(kernel=O.KERNEL_POT_GRAD, convention=1, exclude_self=1, near_stride=stride)
+    dim, _, _, deg, theta, leaf, kernel, _ = wl_spec
+    cfg = O.make_config(dim, deg, leaf, theta, 0.0)
+    o = O.opts(kernel=kernel, convention=1, exclude_self=1, near_stride=stride)
     t0 = time.perf_counter()
-    _, t, _ = O.solve(cfg, o, x, y, None, x, y, None, q)
+    _, t, _ = O.solve(cfg, o, x, y, z, x, y, z, q)
     wall = time.perf_counter() - t0
     near_s, frac = O.last_near()
     est = wall - near_s + near_s / max(frac, 1e-12)
@@ -68,37 +83,37 @@ def cpu_baseline(x, y, q, stride):
         "unit": "targets/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"oracle (C/OpenMP restatement) on the same 1M-point problem: tree, traversal, upward, "
+        "sample": (f"oracle (C/OpenMP restatement) on the same {len(x)}-point problem: tree, traversal, upward, "
                    f"CP/CC and downward over all targets; PP/PC on every {stride}th pair "
                    f"({frac:.3f} of PP/PC evaluations, {near_s:.1f} s), scaled; measured {wall:.1f} s, "
                    f"estimated full solve {est:.1f} s"),
     }
 
 
-def run_variant(btk, torch, cfg, opts, dx, dy, dq, C, n, steps, warmup):
+def run_variant(btk, torch, cfg, opts, d, C, n, steps, warmup):
     """Times `steps` solves after `warmup`; returns (ms per solve, output, stats list)."""
-    out = torch.zeros(C * n, dtype=torch.float64, device=dx.device)
+    out = torch.zeros(C * n, dtype=torch.float64, device=d[0].device)
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         out.zero_()
-        btk.solve(cfg, opts, dx, dy, None, dq, out, stream=stream)
+        btk.solve(cfg, opts, d[0], d[1], d[2], d[3], out, stream=stream)
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
         out.zero_()
-        stats.append(btk.solve(cfg, opts, dx, dy, None, dq, out, stream=stream))
+        stats.append(btk.solve(cfg, opts, d[0], d[1], d[2], d[3], out, stream=stream))
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
     return ms, out, stats
 
 
-def summarize(stats, ms, n, dtype, out, ds):
+def summarize(stats, ms, n, dtype, out, ds, flops_per_eval, idx=None):
     def avg(f):
         return float(np.mean([getattr(s, f) for s in stats]))
     evals = [float(stats[-1].evals[k]) for k in range(4)]
     t_near = avg("t_kernel_near")
-    near_flops = (evals[0] + evals[1]) * FLOPS_PER_EVAL
+    near_flops = (evals[0] + evals[1]) * flops_per_eval
     achieved = near_flops / t_near / 1e12 if t_near > 0 else 0.0
     r = {
         "targets_per_s": n / (ms * 1e-3),
@@ -116,8 +131,14 @@ def summarize(stats, ms, n, dtype, out, ds):
     }
     if ds is not None:
         import torch
-        r["rel_l2"] = float(torch.linalg.norm(out - ds) / torch.linalg.norm(ds))
-        r["rel_l2_potential"] = float(torch.linalg.norm(out[:n] - ds[:n]) / torch.linalg.norm(ds[:n]))
+        o = out.view(-1, n)
+        if idx is not None:
+            o = o[:, idx.long()]
+        o = o.reshape(ds.shape)
+        m = ds.shape[1]
+        r["rel_l2"] = float(torch.linalg.norm(o - ds) / torch.linalg.norm(ds))
+        r["rel_l2_potential"] = float(torch.linalg.norm(o[0] - ds[0]) / torch.linalg.norm(ds[0]))
+        r["rel_l2_targets"] = m
     return r
 
 
@@ -132,26 +153,42 @@ def main():
     from barytreek_amd import workloads as wl
 
     dev = torch.device("cuda:0")
-    x, y, _, q = wl.gaussian_mixture_2d(args.n, SEED)
-    dx, dy, dq = (torch.from_numpy(a).to(dev) for a in (x, y, q))
-    C = btk.components(2, btk.KERNEL_POT_GRAD)
+    spec = WORKLOADS[args.workload]
+    dim, n, gen, degree, theta, leaf, kernel, desc = spec
+    n = args.n or n
+    if gen == "gauss2":
+        x, y, z, q = wl.gaussian_mixture_2d(n, SEED)
+    elif gen == "cube3":
+        x, y, z, q = wl.uniform_cube(n, 3, SEED)
+    else:
+        x, y, z, q = wl.sphere_plummer_3d(n, SEED)
+    d = [None if a is None else torch.from_numpy(a).to(dev) for a in (x, y, z, q)]
+    C = btk.components(dim, kernel)
+    fpe = FLOPS[(dim, kernel)]
     variants = ["plain", "mixed"] if args.variant == "both" else [args.variant]
     prec = {"plain": btk.PREC_FP64, "mixed": btk.PREC_MIXED}
     dtype = {"plain": "f64", "mixed": "f32"}
 
-    ds = None
-    if not args.no_accuracy:  # FP64 direct sum over every target (GPU, checked against the oracle in tests)
-        ds = torch.zeros(C * args.n, dtype=torch.float64, device=dev)
-        btk.direct_sum(2, btk.options(kernel=btk.KERNEL_POT_GRAD, convention=1, exclude_self=1), 0.0, dx, dy, None,
-                       dx, dy, None, dq, ds)
+    # FP64 direct sum (GPU; checked against the oracle in the tests): every
+    # target up to 1M, 10k sampled targets above
+    ds, idx = None, None
+    if not args.no_accuracy:
+        if n > 1_000_000:
+            g = np.random.default_rng(SEED + 1)
+            idx = torch.from_numpy(np.sort(g.choice(n, 10_000, replace=False)).astype(np.int32)).to(dev)
+        m = n if idx is None else int(idx.numel())
+        ds = torch.zeros(C * m, dtype=torch.float64, device=dev)
+        btk.direct_sum(dim, btk.options(kernel=kernel, convention=1, exclude_self=1), 0.0, d[0], d[1], d[2],
+                       d[0], d[1], d[2], d[3], ds, idx=idx)
         torch.cuda.synchronize()
+        ds = ds.view(C, m)
 
     res = {}
     for v in variants:
-        cfg = btk.make_config(2, DEGREE, LEAF, THETA, 0.0)
-        opts = btk.options(kernel=btk.KERNEL_POT_GRAD, convention=1, exclude_self=1, precision=prec[v])
-        ms, out, stats = run_variant(btk, torch, cfg, opts, dx, dy, dq, C, args.n, args.steps, args.warmup)
-        res[v] = summarize(stats, ms, args.n, dtype[v], out, ds)
+        cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
+        opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v])
+        ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
+        res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
         del out
     head = "mixed" if "mixed" in res else variants[0]
     main_r = res[head]
@@ -159,11 +196,12 @@ def main():
     # the same workload at the smallest degree whose error reaches the metric's 1e-6
     at = None
     if ds is not None and not args.no_degree_scan:
-        for deg in range(DEGREE, 11):
-            cfg = btk.make_config(2, deg, LEAF, THETA, 0.0)
-            opts = btk.options(kernel=btk.KERNEL_POT_GRAD, convention=1, exclude_self=1, precision=prec[head])
-            ms, out, stats = run_variant(btk, torch, cfg, opts, dx, dy, dq, C, args.n, args.steps, 1)
-            r = summarize(stats, ms, args.n, dtype[head], out, ds)
+        for deg in range(degree, 11):
+            cfg = btk.make_config(dim, deg, leaf, theta, 0.0)
+            opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[head])
+            ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, 1)
+            r = summarize(stats, ms, n, dtype[head], out, ds, fpe, idx)
+            del out
             if r["rel_l2"] <= 1e-6:
                 at = {"degree": deg, "targets_per_s": r["targets_per_s"], "ms_per_step": ms, "rel_l2": r["rel_l2"],
                       "variant": head}
@@ -182,9 +220,8 @@ def main():
         "vs_baseline": None,
         "dtype": dtype[head],
         "data": "synthetic",
-        "config": {"workload": "2D Poisson -log r/(2pi) potential+gradient, N=M=1,000,000 (4 Gaussians sigma 0.05 "
-                               "+ 20% uniform), q~N(0,1), degree 6, theta 0.6, leaf 1000, eps 0, self excluded",
-                   "variant": head, "n": args.n, "seed": SEED, "parallelism": "single GPU",
+        "config": {"workload": desc, "name": args.workload,
+                   "variant": head, "n": n, "seed": SEED, "parallelism": "single GPU",
                    "accumulation": "f64 per target"},
         "rel_l2": main_r.get("rel_l2"),
         "rel_l2_potential": main_r.get("rel_l2_potential"),
@@ -197,7 +234,7 @@ def main():
     if "plain" in res and "mixed" in res:
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(x, y, q, args.cpu_stride)
+        line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, args.cpu_stride)
         line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
