@@ -319,3 +319,91 @@ inline void poisson_vel_3d_dir_sum(const btkx::DeviceArray<double> &xt, const bt
     btkx::check(btk_direct_sum_3d(&o, eps, (int)xt.size(), nullptr, xt.data(), yt.data(), zt.data(), (int)xs.size(),
                                   xs.data(), ys.data(), zs.data(), charges.data(), vel.data(), nullptr));
 }
+
+// ---------------------------------------------------------------------------
+// On-disk format for fixtures and results ("BTKF", version 1), little endian:
+//   header   char magic[4] = "BTKF"; uint32 version = 1; uint32 n_sections; uint32 0
+//   section  char name[24] (NUL padded); uint32 dtype (0 f64, 1 i32, 2 u8, 3 i64);
+//            uint32 0; uint64 count; count elements, zero padded to 8 bytes
+// Section names used by btk_driver: config (RunConfig bytes), x, y, z, q,
+// panels (TreePanel AoS bytes), level_start, point_leaf_panel, pairs
+// (interact_pair as 3 x i32), soln.  barytreek_amd/fileio.py reads and
+// writes the same format.
+// ---------------------------------------------------------------------------
+#include <cstdint>
+#include <fstream>
+#include <map>
+
+namespace btkx {
+
+struct Section {
+    int dtype = 0;  // 0 f64, 1 i32, 2 u8, 3 i64
+    std::vector<unsigned char> bytes;
+    template <class T>
+    std::vector<T> as() const {
+        std::vector<T> v(bytes.size() / sizeof(T));
+        if (!v.empty()) std::memcpy(v.data(), bytes.data(), v.size() * sizeof(T));
+        return v;
+    }
+};
+
+inline size_t dtype_size(int dtype) { return dtype == 0 || dtype == 3 ? 8 : (dtype == 1 ? 4 : 1); }
+
+template <class T>
+Section make_section(const std::vector<T> &v, int dtype) {
+    Section s;
+    s.dtype = dtype;
+    s.bytes.resize(v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(s.bytes.data(), v.data(), s.bytes.size());
+    return s;
+}
+
+inline void write_file(const std::string &path, const std::map<std::string, Section> &sections) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    const uint32_t head[4] = {0x464b5442u /* "BTKF" */, 1u, (uint32_t)sections.size(), 0u};
+    f.write(reinterpret_cast<const char *>(head), sizeof head);
+    for (const auto &kv : sections) {
+        if (kv.first.size() >= 24) throw std::runtime_error("section name too long: " + kv.first);
+        char name[24] = {0};
+        std::memcpy(name, kv.first.data(), kv.first.size());
+        const uint32_t dt[2] = {(uint32_t)kv.second.dtype, 0u};
+        const uint64_t count = kv.second.bytes.size() / dtype_size(kv.second.dtype);
+        f.write(name, 24);
+        f.write(reinterpret_cast<const char *>(dt), sizeof dt);
+        f.write(reinterpret_cast<const char *>(&count), sizeof count);
+        f.write(reinterpret_cast<const char *>(kv.second.bytes.data()), kv.second.bytes.size());
+        const size_t pad = (8 - kv.second.bytes.size() % 8) % 8;
+        const char zeros[8] = {0};
+        f.write(zeros, pad);
+    }
+    if (!f) throw std::runtime_error("write failed: " + path);
+}
+
+inline std::map<std::string, Section> read_file(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot read " + path);
+    uint32_t head[4];
+    f.read(reinterpret_cast<char *>(head), sizeof head);
+    if (!f || head[0] != 0x464b5442u || head[1] != 1u) throw std::runtime_error("not a BTKF v1 file: " + path);
+    std::map<std::string, Section> out;
+    for (uint32_t i = 0; i < head[2]; i++) {
+        char name[25] = {0};
+        uint32_t dt[2];
+        uint64_t count;
+        f.read(name, 24);
+        f.read(reinterpret_cast<char *>(dt), sizeof dt);
+        f.read(reinterpret_cast<char *>(&count), sizeof count);
+        if (!f || dt[0] > 3) throw std::runtime_error("corrupt section header in " + path);
+        Section s;
+        s.dtype = (int)dt[0];
+        s.bytes.resize(count * dtype_size(s.dtype));
+        f.read(reinterpret_cast<char *>(s.bytes.data()), s.bytes.size());
+        f.ignore((8 - s.bytes.size() % 8) % 8);
+        if (!f) throw std::runtime_error("truncated section " + std::string(name) + " in " + path);
+        out[name] = std::move(s);
+    }
+    return out;
+}
+
+}  // namespace btkx

@@ -39,6 +39,8 @@ struct Args {
     int precision = BTK_PREC_FP64;
     int exclude_self = 0;
     bool direct_sum = true;
+    std::string points;  // BTKF file with sections x, y[, z], q (targets = sources)
+    std::string save;    // BTKF file to write config, points, tree, list and solution into
 };
 
 Args parse(int argc, char **argv) {
@@ -57,6 +59,8 @@ Args parse(int argc, char **argv) {
         else if (k == "--precision") a.precision = next() == "mixed" ? BTK_PREC_MIXED : BTK_PREC_FP64;
         else if (k == "--exclude-self") a.exclude_self = 1;
         else if (k == "--no-direct-sum") a.direct_sum = false;
+        else if (k == "--points") a.points = next();
+        else if (k == "--save") a.save = next();
         else throw std::runtime_error("unknown option " + k);
     }
     return a;
@@ -84,7 +88,16 @@ int run(const Args &a) {
     read_run_config(a.namelist, run_config);
     run_config.mpi_id = 0;
     run_config.mpi_p = 1;
-    const int point_count = a.n > 0 ? a.n : (dim == 3 ? 20000 : 10000);
+    std::map<std::string, btkx::Section> loaded;
+    if (!a.points.empty()) {
+        if (split) throw std::runtime_error("--points is for coincident targets and sources");
+        loaded = btkx::read_file(a.points);
+        for (const char *k : {"x", "y", "q"})
+            if (!loaded.count(k)) throw std::runtime_error(std::string("--points file lacks section ") + k);
+        if (dim == 3 && !loaded.count("z")) throw std::runtime_error("--points file lacks section z");
+    }
+    const int point_count =
+        !a.points.empty() ? (int)loaded["x"].as<double>().size() : (a.n > 0 ? a.n : (dim == 3 ? 20000 : 10000));
     const int kernel = vel ? BTK_KERNEL_REF_VEL3D : BTK_KERNEL_POT;
     const btk_options opts = btkx::options(kernel, a.precision, a.exclude_self);
     const int C = btk_components(dim, kernel);
@@ -104,6 +117,12 @@ int run(const Args &a) {
         } else {  // "whatever, made up" (example_poisson_3d.cpp:67)
             q[i] = 1.0 / (1.0 + x[i] * x[i] + y[i] * y[i] + (dim == 3 ? z[i] * z[i] : 0.0));
         }
+    }
+    if (!a.points.empty()) {
+        x = loaded["x"].as<double>();
+        y = loaded["y"].as<double>();
+        if (dim == 3) z = loaded["z"].as<double>();
+        q = loaded["q"].as<double>();
     }
     DeviceArray<double> dx(x), dy(y), dz(z), dq(q);
     DeviceArray<double> dxs(split ? xs : x), dys(split ? ys : y);
@@ -174,6 +193,42 @@ int run(const Args &a) {
     sync();
     end = clk::now();
     std::printf("downward pass time: %g seconds\n", secs(begin, end));
+
+    if (!a.save.empty()) {
+        std::map<std::string, btkx::Section> out;
+        std::vector<unsigned char> cfg_bytes(sizeof run_config);
+        std::memcpy(cfg_bytes.data(), &run_config, sizeof run_config);
+        out["config"] = btkx::make_section(cfg_bytes, 2);
+        out["x"] = btkx::make_section(x, 0);
+        out["y"] = btkx::make_section(y, 0);
+        if (dim == 3) out["z"] = btkx::make_section(z, 0);
+        out["q"] = btkx::make_section(q, 0);
+        std::vector<unsigned char> panels;
+        if (dim == 3) {
+            std::vector<TreePanel_3d> p(tinfo.panel_count);
+            btkx::check(btk_tree_copy_panels_3d(target.get(), p.data()));
+            panels.resize(p.size() * sizeof(TreePanel_3d));
+            std::memcpy(panels.data(), p.data(), panels.size());
+        } else {
+            std::vector<TreePanel_2d> p(tinfo.panel_count);
+            btkx::check(btk_tree_copy_panels_2d(target.get(), p.data()));
+            panels.resize(p.size() * sizeof(TreePanel_2d));
+            std::memcpy(panels.data(), p.data(), panels.size());
+        }
+        out["panels"] = btkx::make_section(panels, 2);
+        out["level_start"] = btkx::make_section(std::vector<int>(tinfo.level_start,
+                                                                 tinfo.level_start + tinfo.levels + 1), 1);
+        std::vector<int> plp(point_count);
+        btkx::check(btk_tree_copy_point_leaf_panel(target.get(), plp.data()));
+        out["point_leaf_panel"] = btkx::make_section(plp, 1);
+        const std::vector<interact_pair> pairs = interactions.pairs();
+        std::vector<int> flat(pairs.size() * 3);
+        if (!pairs.empty()) std::memcpy(flat.data(), pairs.data(), flat.size() * sizeof(int));
+        out["pairs"] = btkx::make_section(flat, 1);
+        out["soln"] = btkx::make_section(soln.to_host(), 0);
+        btkx::write_file(a.save, out);
+        std::printf("saved %s\n", a.save.c_str());
+    }
 
     if (a.direct_sum) {
         DeviceArray<double> ds((size_t)C * point_count);

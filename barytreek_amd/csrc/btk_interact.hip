@@ -127,6 +127,8 @@ void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const 
     L.excl = o.exclude_self ? 1 : 0;
     L.f32 = f32 ? 1 : 0;
     L.atomic = nsplit > 1 ? 1 : 0;
+    // far targets are proxy points, which no zero-distance rule ties to a panel
+    L.same_tree = (&tt == &ts && !far) ? 1 : 0;
     L.tabs = tabs;
     L.kp = kp;
     L.out = dst;

@@ -85,17 +85,29 @@ __device__ __forceinline__ void eval_pair(const Src64 &s, const double *t, doubl
         eval_f64<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc64);
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin.  Work items are
+// ordered by target panel (neighbouring leaves share most of their
+// sources), so each XCD is given one contiguous range of them and its L2
+// serves the shared source tiles.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous(int bid, int n) {
+    const int x = bid % kXcds, r = bid / kXcds;
+    const int per = n / kXcds, extra = n % kXcds;
+    return x * per + min(x, extra) + r;
+}
+
 template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL>
 __global__ void __launch_bounds__(kLanes)
     k_interact(TaskView tk, GroupView gv, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
-               double *__restrict__ out, size_t comp_stride, int atomic) {
+               double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     __shared__ Src64 tile[kLanes];
     __shared__ double cn[3][kMaxN1];
     const int lane = threadIdx.x;
-    const int T = tk.panel[blockIdx.x];
-    const int first = tk.first[blockIdx.x];
-    const int e0 = tk.e0[blockIdx.x], e1 = tk.e1[blockIdx.x];
+    const int item = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int T = tk.panel[item];
+    const int first = tk.first[item];
+    const int e0 = tk.e0[item], e1 = tk.e1[item];
     const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
@@ -129,6 +141,10 @@ __global__ void __launch_bounds__(kLanes)
             }
             __syncthreads();
         }
+        // With one tree for targets and sources, coincident points share a leaf,
+        // so among point sources only the target leaf's own entry can hold a
+        // zero-distance pair.  Proxy sources keep the test.
+        const bool check = !same_tree || S == T || cluster;
         const int ss = cluster ? 0 : sv.start[S];
         const double *w = sv.W + (size_t)S * P;
         for (int t0 = 0; t0 < nsrc; t0 += kLanes) {
@@ -155,7 +171,15 @@ __global__ void __launch_bounds__(kLanes)
             for (int k = 0; k < TPT; k++)
 #pragma unroll
                 for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
-            if (nt == kLanes) {
+            if (nt == kLanes && EXCL && !check) {
+                // no zero-distance pair is possible here: skip the exclusion test
+#pragma unroll 8
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Src64 s = tile[jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval_pair<DIM, KERNEL, F32, false>(s, t[k], kp.eps2, acc[k], a32[k]);
+                }
+            } else if (nt == kLanes) {
 #pragma unroll 8
                 for (int jj = 0; jj < kLanes; jj++) {
                     const Src64 s = tile[jj];
@@ -205,6 +229,7 @@ struct InteractLaunch {
     TgtView tv;
     SrcView sv;
     int n_tasks, n1, tpt, far, excl, f32, atomic;
+    int same_tree;  // targets and sources are one tree (the far group never compares panels)
     NodeTables tabs;
     KParams kp;
     double *out;
@@ -221,7 +246,7 @@ namespace btk {
 #define BTK_IK(F32_, TPT_, FAR_, EXCL_)                                                                          \
     k_interact<DIM, KERNEL, F32_, TPT_, FAR_, EXCL_><<<L.n_tasks, kLanes, 0, s>>>(L.tk, L.gv, L.tv, L.sv, L.n1,  \
                                                                                  L.tabs, L.kp, L.out,            \
-                                                                                 L.comp_stride, L.atomic)
+                                                                                 L.comp_stride, L.atomic, L.same_tree)
 #define BTK_IK_FE(F32_, TPT_)                                  \
     do {                                                       \
         if (L.far) {                                           \

@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (scale for the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -153,6 +155,14 @@ def main():
     from barytreek_amd import workloads as wl
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_calibration}), flush=True)
+        return
     spec = WORKLOADS[args.workload]
     dim, n, gen, degree, theta, leaf, kernel, desc = spec
     n = args.n or n
@@ -231,6 +241,13 @@ def main():
         "at_rel_l2_1e-6": at,
         "cpu_baseline": None,
     }
+    # HBM bytes per launch of the near kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size)
+    tfile = os.path.join(HERE, "profiles", f"traffic_k_interact_near_{args.workload}_{head}.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        line["roofline"] = dict(line["roofline"], traffic=t["traffic_bytes_per_launch"],
+                                traffic_source=os.path.relpath(tfile, HERE))
     if "plain" in res and "mixed" in res:
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
     if not args.no_cpu_baseline:
