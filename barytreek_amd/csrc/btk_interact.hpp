@@ -87,13 +87,17 @@ __device__ __forceinline__ void eval_pair(const Src64 &s, const double *t, doubl
 
 // Workgroups are dispatched to the 8 XCDs round robin.  Work items are
 // ordered by target panel (neighbouring leaves share most of their
-// sources), so each XCD is given one contiguous range of them and its L2
-// serves the shared source tiles.
+// sources), so each XCD takes runs of kRun consecutive items -- its L2 then
+// serves the shared source tiles -- and the runs go round robin, which keeps
+// dense and sparse regions spread over all XCDs.  Items past the last full
+// round keep their own index.
 constexpr int kXcds = 8;
+constexpr int kRun = 16;
 __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
+    const int full = n / (kXcds * kRun) * (kXcds * kRun);
+    if (bid >= full) return bid;
     const int x = bid % kXcds, r = bid / kXcds;
-    const int per = n / kXcds, extra = n % kXcds;
-    return x * per + min(x, extra) + r;
+    return (r / kRun) * (kXcds * kRun) + x * kRun + (r % kRun);
 }
 
 template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL>
