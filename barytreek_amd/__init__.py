@@ -31,7 +31,7 @@ EXPORTED = [
     "btk_dual_tree_traversal_2d", "btk_dual_tree_traversal_3d", "btk_list_count", "btk_list_copy",
     "btk_split_interactions", "btk_list_destroy", "btk_upward_pass_2d", "btk_upward_pass_3d",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
-    "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve",
+    "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi",
 ]
 
 
@@ -128,6 +128,8 @@ def lib():
     L.btk_direct_sum_3d.argtypes = [optp, C.c_double, C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, vp]
     L.btk_solve.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, dp,
                             C.POINTER(SolveStats), vp]
+    L.btk_solve_multi.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, dp,
+                                  C.POINTER(SolveStats), vp]
     _lib = L
     return L
 
@@ -344,4 +346,17 @@ def solve(cfg, opts, xt, yt, zt, q, out, xs=None, ys=None, zs=None, stream=None)
     _check(lib().btk_solve(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
                            int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), int(coincident), _ptr(out),
                            C.byref(st), _stream(stream)))
+    return st
+
+
+def solve_multi(cfg, opts, xt, yt, zt, q, out, xs=None, ys=None, zs=None, stream=None) -> SolveStats:
+    """Many charge vectors over one geometry: q is [K][n], out [K][C][M] (added to)."""
+    coincident = xs is None
+    if coincident:
+        xs, ys, zs = xt, yt, zt
+    K = int(q.shape[0])
+    st = SolveStats()
+    _check(lib().btk_solve_multi(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
+                                 int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), K, int(coincident),
+                                 _ptr(out), C.byref(st), _stream(stream)))
     return st

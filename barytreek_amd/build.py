@@ -30,6 +30,7 @@ SOURCES = [
     "btk_ik_3d_pot.hip",
     "btk_ik_3d_vel.hip",
     "btk_ik_3d_grad.hip",
+    "btk_mrhs.hip",
     "btk_api.hip",
 ]
 

@@ -492,4 +492,86 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
     });
 }
 
+int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
+                    const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                    const double *d_zs, const double *d_q, int nrhs, int coincident, double *d_out,
+                    btk_solve_stats *stats, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in, "null run config");
+        BTK_REQUIRE(nrhs >= 1, "nrhs must be positive");
+        btk_run_config cfg = *cfg_in;
+        const int dim = cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        const btk_options o = opts_or_default(opts);
+        BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "btk_solve_multi: potential or potential+gradient kernels");
+        BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
+        hipStream_t s = as_stream(stream);
+        using clk = std::chrono::steady_clock;
+        auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto t0 = clk::now();
+        auto ts = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
+        std::unique_ptr<btk::Tree> tt_own;
+        if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
+        const btk::Tree &tt = coincident ? *ts : *tt_own;
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t1 = clk::now();
+        auto l = btk::build_list(&cfg, tt, *ts, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t2 = clk::now();
+        const int P = btk::ipow(cfg.interp_degree + 1, dim);
+        const int C = btk::components(dim, o.kernel);
+        const size_t wrows = (size_t)ts->pc * P, prow = (size_t)C * tt.pc * P;
+        btk::DBuf<double> q((size_t)nrhs * (n ? n : 1)), W((size_t)nrhs * wrows), Psi((size_t)nrhs * prow);
+        for (int r = 0; r < nrhs; r++) {
+            btk::gather_sorted(*ts, d_q + (size_t)r * n, q.p + (size_t)r * n, s);
+            btk::upward_pass(cfg, *ts, q.p + (size_t)r * n, W.p + (size_t)r * wrows, s);
+        }
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t3 = clk::now();
+        BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
+        btk::KernelEvents kev;
+        for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
+        if (o.precision == BTK_PREC_MIXED && nrhs >= 16) {
+            const int kp = (nrhs + 63) / 64 * 64;
+            btk::DBuf<float> qs((size_t)(n ? n : 1) * kp), wt(wrows * kp);
+            btk::gather_multi_f32(*ts, d_q, nrhs, kp, qs.p, s);
+            btk::transpose_weights_f32((long long)wrows, nrhs, kp, W.p, wt.p, s);
+            btk::fmm_interactions_multi(cfg, o, tt, *ts, *l, qs.p, wt.p, nrhs, kp, d_out, Psi.p, s, &kev);
+            BTK_HIP(hipStreamSynchronize(s));
+        } else {
+            for (int r = 0; r < nrhs; r++)
+                btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p + (size_t)r * n, W.p + (size_t)r * wrows,
+                                      d_out + (size_t)r * C * m, Psi.p + (size_t)r * prow, s, r == 0 ? &kev : nullptr);
+            BTK_HIP(hipStreamSynchronize(s));
+        }
+        const auto t4 = clk::now();
+        float ms_near = 0.f, ms_far = 0.f;
+        if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
+        if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
+        for (auto &e : kev.ev) (void)hipEventDestroy(e);
+        for (int r = 0; r < nrhs; r++)
+            btk::downward_pass(cfg, o, tt, Psi.p + (size_t)r * prow, d_out + (size_t)r * C * m, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t5 = clk::now();
+        if (stats) {
+            stats->t_tree = sec(t0, t1);
+            stats->t_traversal = sec(t1, t2);
+            stats->t_upward = sec(t2, t3);
+            stats->t_interact = sec(t3, t4);
+            stats->t_downward = sec(t4, t5);
+            stats->t_total = sec(t0, t5);
+            for (int k = 0; k < 4; k++) {
+                stats->count[k] = l->counts[k];
+                stats->evals[k] = l->evals[k];
+            }
+            stats->t_kernel_near = 1e-3 * ms_near;
+            stats->t_kernel_far = 1e-3 * ms_far;
+            stats->panel_count_target = tt.pc;
+            stats->panel_count_source = ts->pc;
+            stats->levels_target = tt.levels;
+            stats->levels_source = ts->levels;
+        }
+    });
+}
+
 }  // extern "C"

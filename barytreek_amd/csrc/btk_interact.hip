@@ -66,39 +66,6 @@ __global__ void k_task_fill(int pct, const int *__restrict__ goff, int far, cons
     }
 }
 
-struct Tasks {
-    int n = 0;
-    DBuf<int> panel, first, e0, e1;
-};
-
-void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
-                 hipStream_t s) {
-    const int pct = tt.pc;
-    DBuf<int> nt(pct), tb(pct);
-    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, tt.count.p, P, per, nsplit, nt.p);
-    BTK_LAUNCH_CHECK();
-    exclusive_sum_i32(nt.p, tb.p, pct, s);
-    int *h = reinterpret_cast<int *>(host_scratch().v);
-    BTK_HIP(hipMemcpyAsync(h, tb.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    BTK_HIP(hipMemcpyAsync(h + 1, nt.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    BTK_HIP(hipStreamSynchronize(s));
-    tk.n = h[0] + h[1];
-    const size_t m = tk.n ? tk.n : 1;
-    tk.panel.resize(m);
-    tk.first.resize(m);
-    tk.e0.resize(m);
-    tk.e1.resize(m);
-    k_task_fill<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit,
-                                         tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
-    BTK_LAUNCH_CHECK();
-}
-
-int choose_split(long long base_items) {
-    if (base_items <= 0) return 1;
-    long long k = (kTargetWaves + base_items - 1) / base_items;
-    return (int)std::max(1ll, std::min<long long>(kMaxSplit, k));
-}
-
 template <int DIM, int KERNEL>
 void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt,
                const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, double *dst, hipStream_t s,
@@ -201,6 +168,34 @@ void launch_direct(const KParams &kp, int m, const int *idx, const double *xt, c
 }
 
 }  // namespace
+
+void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
+                 hipStream_t s) {
+    const int pct = tt.pc;
+    DBuf<int> nt(pct), tb(pct);
+    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, tt.count.p, P, per, nsplit, nt.p);
+    BTK_LAUNCH_CHECK();
+    exclusive_sum_i32(nt.p, tb.p, pct, s);
+    int *h = reinterpret_cast<int *>(host_scratch().v);
+    BTK_HIP(hipMemcpyAsync(h, tb.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipMemcpyAsync(h + 1, nt.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    tk.n = h[0] + h[1];
+    const size_t m = tk.n ? tk.n : 1;
+    tk.panel.resize(m);
+    tk.first.resize(m);
+    tk.e0.resize(m);
+    tk.e1.resize(m);
+    k_task_fill<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit,
+                                         tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
+    BTK_LAUNCH_CHECK();
+}
+
+int choose_split(long long base_items) {
+    if (base_items <= 0) return 1;
+    long long k = (kTargetWaves + base_items - 1) / base_items;
+    return (int)std::max(1ll, std::min<long long>(kMaxSplit, k));
+}
 
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts, const List &l,
                       const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s,

@@ -226,6 +226,17 @@ __global__ void __launch_bounds__(kLanes)
     }
 }
 
+// Work items of one group (near: PP+PC, far: CP+CC): blocks of `per`
+// targets of a panel times slices of its entries, balanced by source count.
+struct Tasks {
+    int n = 0;
+    DBuf<int> panel, first, e0, e1;
+};
+void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
+                 hipStream_t s);
+// Entry slices per panel so that about 8 wavefronts per SIMD are in flight.
+int choose_split(long long base_items);
+
 // Launch of one interaction group with everything fixed but the kernel.
 struct InteractLaunch {
     TaskView tk;

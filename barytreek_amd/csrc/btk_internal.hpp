@@ -181,6 +181,17 @@ void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tre
                       hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
                    hipStream_t s);
+
+// Many charge vectors (btk_mrhs.hip).  q is [K][n] in original order;
+// qs is [n][kp] FP32 in sorted order, kp = K rounded up to 64, zero padded.
+void gather_multi_f32(const Tree &t, const double *q, int K, int kp, float *qs, hipStream_t s);
+// W [K][rows] FP64 -> Wt [rows][kp] FP32
+void transpose_weights_f32(long long rows, int K, int kp, const double *W, float *Wt, hipStream_t s);
+// FP32 MFMA interactions of K vectors: adds PP+PC into out [K][C][M] and
+// CP+CC into Psi [K][C][pc_t][P].
+void fmm_interactions_multi(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
+                            const List &l, const float *qs, const float *wt, int K, int kp, double *out, double *Psi,
+                            hipStream_t s, KernelEvents *kev);
 void direct_sum(int dim, const btk_options &o, double eps, int m, const int *idx, const double *xt,
                 const double *yt, const double *zt, int n, const double *xs, const double *ys,
                 const double *zs, const double *q, double *out, hipStream_t s);

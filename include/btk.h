@@ -209,6 +209,17 @@ int btk_solve(const btk_run_config *cfg, const btk_options *opts, int m, const d
               const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
               const double *d_q, int coincident, double *d_out, btk_solve_stats *stats, void *stream);
 
+/* Many charge vectors over one geometry (no reference counterpart): btk_solve
+ * for nrhs vectors d_q[nrhs][n] sharing one tree, list and set of kernel
+ * evaluations.  Adds into d_out[nrhs][C][m].  With BTK_PREC_MIXED and
+ * nrhs >= 16 the interactions run as FP32 MFMA products (kernel tile x charge
+ * tile); otherwise each vector runs the single-vector interaction kernels.
+ * Kernels: BTK_KERNEL_POT or BTK_KERNEL_POT_GRAD. */
+int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
+                    const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
+                    const double *d_q, int nrhs, int coincident, double *d_out, btk_solve_stats *stats,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif
