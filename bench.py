@@ -46,6 +46,14 @@ WORKLOADS = {
 }
 
 
+# name: (dim, n, degree, theta, leaf, charge vectors, description)
+BATCHED = {
+    "3d-batched-2M": (3, 2_000_000, 8, 0.7, 2000, 64,
+                      "3D Poisson 1/(4pi r) potential, N=M=2,000,000 uniform in [-1,1]^3, 64 charge vectors "
+                      "N(0,1) (FP32-representable), degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,7 +61,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", choices=["both", "plain", "mixed"], default="both")
     ap.add_argument("--no-degree-scan", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED), default="2d")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -144,6 +152,79 @@ def summarize(stats, ms, n, dtype, out, ds, flops_per_eval, idx=None):
     return r
 
 
+def bench_batched(args, btk, torch, wl, dev):
+    """Many charge vectors over one geometry: K vectors solved together
+    (btk_solve_multi; FP32 MFMA products in the mixed variant) against the
+    same K solved one by one in FP64 (timed on a few vectors, scaled)."""
+    dim, n, degree, theta, leaf, K, desc = BATCHED[args.workload]
+    n = args.n or n
+    x, y, z, _ = wl.uniform_cube(n, 3, SEED)
+    rng = np.random.default_rng(SEED + 7)
+    Q = rng.standard_normal((K, n)).astype(np.float32).astype(np.float64)  # FP32-representable charges
+    d = [torch.from_numpy(a).to(dev) for a in (x, y, z)]
+    dq = torch.from_numpy(Q).to(dev)
+    cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
+    kernel = btk.KERNEL_POT
+
+    def timed(prec, k, steps, warmup):
+        out = torch.zeros(k * n, dtype=torch.float64, device=dev)
+        qk = dq[:k].contiguous()
+        o = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec)
+        for _ in range(warmup):
+            out.zero_()
+            btk.solve_multi(cfg, o, d[0], d[1], d[2], qk, out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = None
+        for _ in range(steps):
+            out.zero_()
+            st = btk.solve_multi(cfg, o, d[0], d[1], d[2], qk, out)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out.view(k, n), st
+
+    t_mixed, out_m, st = timed(btk.PREC_MIXED, K, args.steps, args.warmup)
+    kp = min(4, K)
+    t_plain_k, out_p, _ = timed(btk.PREC_FP64, kp, 1, 1)
+    t_plain = t_plain_k * K / kp
+    # accuracy: 4 vectors, 10k sampled targets, FP64 direct sum on the GPU
+    g = np.random.default_rng(SEED + 1)
+    idx = torch.from_numpy(np.sort(g.choice(n, 10_000, replace=False)).astype(np.int32)).to(dev)
+    num = den = num_p = 0.0
+    for r in range(kp):
+        ds = torch.zeros(idx.numel(), dtype=torch.float64, device=dev)
+        btk.direct_sum(3, btk.options(kernel=kernel, convention=1, exclude_self=1), 0.0, d[0], d[1], d[2],
+                       d[0], d[1], d[2], dq[r].contiguous(), ds, idx=idx)
+        num += float(torch.sum((out_m[r][idx.long()] - ds) ** 2))
+        num_p += float(torch.sum((out_p[r][idx.long()] - ds) ** 2))
+        den += float(torch.sum(ds ** 2))
+    evals = [float(st.evals[k]) for k in range(4)]
+    line = {
+        "metric": "targets/s at <=1e-6 rel-L2 vs FP64 direct sum; plain vs mixed/batched/learned",
+        "value": K * n / t_mixed,
+        "unit": "targets/s (target x charge-vector solves per second)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "name": args.workload, "n": n, "charge_vectors": K, "seed": SEED,
+                   "parallelism": "single GPU", "variant": "batched (FP32 MFMA products, FP64 accumulation)"},
+        "rel_l2": (num / den) ** 0.5, "rel_l2_plain": (num_p / den) ** 0.5, "rel_l2_targets": int(idx.numel()),
+        "batched_over_plain": t_plain / t_mixed,
+        "plain_ms_per_step_scaled": t_plain * 1e3,
+        "phases_s": {k: getattr(st, "t_" + k) for k in ("tree", "traversal", "upward", "interact", "downward",
+                                                         "total")},
+        "t_kernel_near_s": st.t_kernel_near, "t_kernel_far_s": st.t_kernel_far,
+        "evals_per_vector": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
+        "cpu_baseline": None,
+    }
+    # near kernel: 2 flops per (target, source, vector) product on MFMA
+    prod = (evals[0] + evals[1]) * K * 2
+    if st.t_kernel_near > 0:
+        a = prod / st.t_kernel_near / 1e12
+        line["roofline"] = {"bound": "mfma", "achieved": round(a, 3), "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                            "frac": round(a / PEAK_TFLOPS["f32"], 4), "traffic": None,
+                            "kernel": "k_interact_mrhs (near, MFMA G x Q)", "avg_launch_s": st.t_kernel_near}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -163,6 +244,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": args.copy_calibration}), flush=True)
         return
+    if args.workload in BATCHED:
+        return bench_batched(args, btk, torch, wl, dev)
     spec = WORKLOADS[args.workload]
     dim, n, gen, degree, theta, leaf, kernel, desc = spec
     n = args.n or n
