@@ -31,7 +31,7 @@ EXPORTED = [
     "btk_dual_tree_traversal_2d", "btk_dual_tree_traversal_3d", "btk_list_count", "btk_list_copy",
     "btk_split_interactions", "btk_list_destroy", "btk_upward_pass_2d", "btk_upward_pass_3d",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
-    "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi",
+    "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
 ]
 
 
@@ -130,6 +130,7 @@ def lib():
                             C.POINTER(SolveStats), vp]
     L.btk_solve_multi.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, dp,
                                   C.POINTER(SolveStats), vp]
+    L.btk_solve_batch.argtypes = [cfgp, optp, C.c_int, ip, dp, dp, dp, dp, dp, C.POINTER(SolveStats), vp]
     _lib = L
     return L
 
@@ -358,5 +359,16 @@ def solve_multi(cfg, opts, xt, yt, zt, q, out, xs=None, ys=None, zs=None, stream
     st = SolveStats()
     _check(lib().btk_solve_multi(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
                                  int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), K, int(coincident),
+                                 _ptr(out), C.byref(st), _stream(stream)))
+    return st
+
+
+def solve_batch(cfg, opts, offsets, x, y, z, q, out, stream=None) -> SolveStats:
+    """Many independent problems in one solve: problem p owns points
+    offsets[p]:offsets[p+1]; adds into out[C][N]."""
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    st = SolveStats()
+    _check(lib().btk_solve_batch(C.byref(cfg), C.byref(opts), len(off) - 1,
+                                 off.ctypes.data_as(C.POINTER(C.c_int)), _ptr(x), _ptr(y), _ptr(z), _ptr(q),
                                  _ptr(out), C.byref(st), _stream(stream)))
     return st

@@ -492,6 +492,72 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
     });
 }
 
+int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int nprob, const int *offsets,
+                    const double *d_x, const double *d_y, const double *d_z, const double *d_q, double *d_out,
+                    btk_solve_stats *stats, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in && offsets, "null argument");
+        BTK_REQUIRE(nprob >= 1, "nprob must be positive");
+        btk_run_config cfg = *cfg_in;
+        const int dim = cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        const btk_options o = opts_or_default(opts);
+        hipStream_t s = as_stream(stream);
+        const std::vector<int> off(offsets, offsets + nprob + 1);
+        BTK_REQUIRE(off[0] == 0, "offsets[0] must be 0");
+        for (int p = 0; p < nprob; p++) BTK_REQUIRE(off[p + 1] > off[p], "every problem needs at least one point");
+        // deepest_only reproduces the reference's level-of-the-last-leaf rule,
+        // which is a property of one tree, not of a forest
+        BTK_REQUIRE(!(o.deepest_only && nprob > 1), "deepest_only is defined for a single problem only");
+        using clk = std::chrono::steady_clock;
+        auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto t0 = clk::now();
+        auto t = btk::build_forest(dim, cfg.fmm_cluster_thresh, d_x, d_y, d_z, off, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t1 = clk::now();
+        auto l = btk::build_list(&cfg, *t, *t, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t2 = clk::now();
+        const int P = btk::ipow(cfg.interp_degree + 1, dim);
+        const int C = btk::components(dim, o.kernel);
+        const int n = t->n;
+        btk::DBuf<double> q(n ? n : 1), W((size_t)t->pc * P), Psi((size_t)C * t->pc * P);
+        btk::gather_sorted(*t, d_q, q.p, s);
+        btk::upward_pass(cfg, *t, q.p, W.p, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t3 = clk::now();
+        BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
+        btk::KernelEvents kev;
+        for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
+        btk::fmm_interactions(cfg, o, *t, *t, *l, q.p, W.p, d_out, Psi.p, s, &kev);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t4 = clk::now();
+        float ms_near = 0.f, ms_far = 0.f;
+        if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
+        if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
+        for (auto &e : kev.ev) (void)hipEventDestroy(e);
+        btk::downward_pass(cfg, o, *t, Psi.p, d_out, s);
+        BTK_HIP(hipStreamSynchronize(s));
+        const auto t5 = clk::now();
+        if (stats) {
+            stats->t_tree = sec(t0, t1);
+            stats->t_traversal = sec(t1, t2);
+            stats->t_upward = sec(t2, t3);
+            stats->t_interact = sec(t3, t4);
+            stats->t_downward = sec(t4, t5);
+            stats->t_total = sec(t0, t5);
+            for (int k = 0; k < 4; k++) {
+                stats->count[k] = l->counts[k];
+                stats->evals[k] = l->evals[k];
+            }
+            stats->t_kernel_near = 1e-3 * ms_near;
+            stats->t_kernel_far = 1e-3 * ms_far;
+            stats->panel_count_target = stats->panel_count_source = t->pc;
+            stats->levels_target = stats->levels_source = t->levels;
+        }
+    });
+}
+
 int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
                     const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
                     const double *d_zs, const double *d_q, int nrhs, int coincident, double *d_out,

@@ -109,6 +109,7 @@ struct Tree {
     int dim = 3;
     int n = 0;
     int thresh = 0;
+    int nprob = 1;   // independent problems: panels 0..nprob-1 are their roots (level 0)
     int pc = 0;      // panel count
     int levels = 0;
     int cap = 0;     // panel capacity of the device arrays
@@ -167,6 +168,11 @@ int components(int dim, int kernel);
 // Phases.
 std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const double *y, const double *z, int n,
                                  hipStream_t s);
+// A forest: problem p owns points [offsets[p], offsets[p+1]) and root panel p;
+// each problem's tree is the tree build_tree gives for its points alone, with
+// panel ids interleaved level by level across the problems.
+std::unique_ptr<Tree> build_forest(int dim, int thresh, const double *x, const double *y, const double *z,
+                                   const std::vector<int> &offsets, hipStream_t s);
 std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &t, const Tree &s, hipStream_t st);
 void upward_pass(const btk_run_config &cfg, const Tree &src, const double *q_sorted, double *W, hipStream_t s);
 void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_t s);

@@ -155,10 +155,13 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
     const TreeView T = view(tt), S = view(ts);
     long long *hv = host_scratch().v;
 
-    DBuf<int2> cur(1), nxt;
-    const int2 root = make_int2(0, 0);
-    BTK_HIP(hipMemcpyAsync(cur.p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
-    int G = 1;
+    // generation 0: the root pair of every problem (one pair for one tree)
+    BTK_REQUIRE(tt.nprob == ts.nprob, "target and source forests hold different numbers of problems");
+    std::vector<int2> roots(tt.nprob);
+    for (int p = 0; p < tt.nprob; p++) roots[p] = make_int2(p, p);
+    DBuf<int2> cur(tt.nprob), nxt;
+    BTK_HIP(hipMemcpyAsync(cur.p, roots.data(), sizeof(int2) * tt.nprob, hipMemcpyHostToDevice, s));
+    int G = tt.nprob;
     int L = 0;
     size_t lcap = 1024;
     l->pairs.resize(lcap);

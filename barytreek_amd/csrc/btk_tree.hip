@@ -76,13 +76,15 @@ __device__ void block_reduce_box(BoxAcc &a, BoxAcc *sh) {
     a = sh[0];
 }
 
+// Chunk c covers points [cb[c], ce[c]) of one problem.
 __global__ void __launch_bounds__(TB) k_box_partial(int dim, const double *__restrict__ x, const double *__restrict__ y,
-                                                    const double *__restrict__ z, int n, BoxAcc *__restrict__ part) {
+                                                    const double *__restrict__ z, const int *__restrict__ cb,
+                                                    const int *__restrict__ ce, BoxAcc *__restrict__ part) {
     __shared__ BoxAcc sh[TB];
     BoxAcc a;
     box_init(a);
     const double *co[3] = {x, y, z};
-    for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) {
+    for (int i = cb[blockIdx.x] + threadIdx.x; i < ce[blockIdx.x]; i += TB) {
         for (int d = 0; d < dim; d++) {
             double v = co[d][i];
             if (better_min(v, i, a.mn[d], a.imn[d])) {
@@ -99,32 +101,35 @@ __global__ void __launch_bounds__(TB) k_box_partial(int dim, const double *__res
     if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
-__global__ void __launch_bounds__(TB) k_box_final(int dim, const BoxAcc *__restrict__ part, int nparts, int n,
+// Root panel p of the forest (one per problem) from chunks [pc0[p], pc0[p+1]).
+__global__ void __launch_bounds__(TB) k_box_final(int dim, const BoxAcc *__restrict__ part,
+                                                  const int *__restrict__ pc0, const int *__restrict__ off,
                                                   double *lo0, double *lo1, double *lo2, double *hi0, double *hi1,
                                                   double *hi2, int *count, int *start, int *parent, int *child0,
                                                   int *level) {
     __shared__ BoxAcc sh[TB];
+    const int p = blockIdx.x;
     BoxAcc a;
     box_init(a);
-    for (int i = threadIdx.x; i < nparts; i += TB) box_merge(a, part[i]);
+    for (int i = pc0[p] + threadIdx.x; i < pc0[p + 1]; i += TB) box_merge(a, part[i]);
     block_reduce_box(a, sh);
     if (threadIdx.x == 0) {
         double *lo[3] = {lo0, lo1, lo2};
         double *hi[3] = {hi0, hi1, hi2};
         for (int d = 0; d < 3; d++) {
             if (d < dim) {
-                lo[d][0] = a.mn[d];                            // DBL_MAX when no point
-                hi[d][0] = (DBL_MIN < a.mx[d]) ? a.mx[d] : DBL_MIN;  // std::max(DBL_MIN, ...)
+                lo[d][p] = a.mn[d];                                   // DBL_MAX when no point
+                hi[d][p] = (DBL_MIN < a.mx[d]) ? a.mx[d] : DBL_MIN;  // std::max(DBL_MIN, ...)
             } else {
-                lo[d][0] = 0.0;
-                hi[d][0] = 0.0;
+                lo[d][p] = 0.0;
+                hi[d][p] = 0.0;
             }
         }
-        count[0] = n;
-        start[0] = 0;
-        parent[0] = -1;
-        child0[0] = -1;
-        level[0] = 0;
+        count[p] = off[p + 1] - off[p];
+        start[p] = off[p];
+        parent[p] = -1;
+        child0[p] = -1;
+        level[p] = 0;
     }
 }
 
@@ -472,14 +477,23 @@ void build_levels(Tree &t, hipStream_t s) {
 
 std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const double *y, const double *z, int n,
                                  hipStream_t s) {
+    return build_forest(dim, thresh, x, y, z, std::vector<int>{0, n}, s);
+}
+
+std::unique_ptr<Tree> build_forest(int dim, int thresh, const double *x, const double *y, const double *z,
+                                   const std::vector<int> &offsets, hipStream_t s) {
     BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
-    BTK_REQUIRE(n >= 0, "negative point count");
+    BTK_REQUIRE(offsets.size() >= 2 && offsets[0] == 0, "problem offsets must start at 0");
+    const int nprob = (int)offsets.size() - 1;
+    for (int p = 0; p < nprob; p++) BTK_REQUIRE(offsets[p + 1] >= offsets[p], "problem offsets must not decrease");
+    const int n = offsets.back();
     auto t = std::make_unique<Tree>();
     t->dim = dim;
     t->n = n;
     t->thresh = thresh;
-    t->pc = 1;
-    grow_panels(*t, 64, s);
+    t->nprob = nprob;
+    t->pc = nprob;
+    grow_panels(*t, std::max(64, nprob), s);
     const size_t nn = n ? n : 1;
     t->perm.resize(nn);
     t->point_leaf.resize(nn);
@@ -493,14 +507,32 @@ std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const dou
         k_iota<<<nblk(n), TB, 0, s>>>(n, t->perm.p, t->point_leaf.p);
         BTK_LAUNCH_CHECK();
     }
-    const int nparts = n ? std::min(1024, nblk(n)) : 1;
+    // root boxes: chunks of at most 64k points per problem, then one block per problem
+    std::vector<int> cb, ce, pc0(1, 0);
+    const int chunk = std::max(65536, n / 1024 + 1);
+    for (int p = 0; p < nprob; p++) {
+        int b = offsets[p];
+        do {
+            cb.push_back(b);
+            ce.push_back(std::min(offsets[p + 1], b + chunk));
+            b += chunk;
+        } while (b < offsets[p + 1]);
+        pc0.push_back((int)cb.size());
+    }
+    const int nparts = (int)cb.size();
+    DBuf<int> dcb(nparts), dce(nparts), dpc0(nprob + 1), doff(nprob + 1);
     DBuf<BoxAcc> part(nparts);
-    k_box_partial<<<nparts, TB, 0, s>>>(dim, x, y, dim == 3 ? z : x, n, part.p);
+    BTK_HIP(hipMemcpyAsync(dcb.p, cb.data(), sizeof(int) * nparts, hipMemcpyHostToDevice, s));
+    BTK_HIP(hipMemcpyAsync(dce.p, ce.data(), sizeof(int) * nparts, hipMemcpyHostToDevice, s));
+    BTK_HIP(hipMemcpyAsync(dpc0.p, pc0.data(), sizeof(int) * (nprob + 1), hipMemcpyHostToDevice, s));
+    BTK_HIP(hipMemcpyAsync(doff.p, offsets.data(), sizeof(int) * (nprob + 1), hipMemcpyHostToDevice, s));
+    k_box_partial<<<nparts, TB, 0, s>>>(dim, x, y, dim == 3 ? z : x, dcb.p, dce.p, part.p);
     BTK_LAUNCH_CHECK();
-    k_box_final<<<1, TB, 0, s>>>(dim, part.p, nparts, n, t->lo[0].p, t->lo[1].p, t->lo[2].p, t->hi[0].p,
-                                 t->hi[1].p, t->hi[2].p, t->count.p, t->start.p, t->parent.p, t->child0.p,
-                                 t->level.p);
+    k_box_final<<<nprob, TB, 0, s>>>(dim, part.p, dpc0.p, doff.p, t->lo[0].p, t->lo[1].p, t->lo[2].p, t->hi[0].p,
+                                     t->hi[1].p, t->hi[2].p, t->count.p, t->start.p, t->parent.p, t->child0.p,
+                                     t->level.p);
     BTK_LAUNCH_CHECK();
+    BTK_HIP(hipStreamSynchronize(s));  // the host vectors above are released on return
     if (dim == 3)
         build_levels<3>(*t, s);
     else

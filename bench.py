@@ -53,6 +53,14 @@ BATCHED = {
                       "N(0,1) (FP32-representable), degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
 }
 
+# name: (dim, problems, min size, max size, plummer a, degree, theta, leaf, description)
+FOREST = {
+    "3d-forest-256": (3, 256, 2000, 200_000, 0.5, 8, 0.7, 500,
+                      "256 independent 3D Poisson 1/(4pi r) potential problems, sizes log-uniform in "
+                      "[2000, 200000], each a Plummer sphere a=0.5 truncated at r<=5, q~U[-1,1], degree 8, "
+                      "theta 0.7, leaf 500, eps 0, self excluded; all problems in one forest solve"),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -61,7 +69,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", choices=["both", "plain", "mixed"], default="both")
     ap.add_argument("--no-degree-scan", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED), default="2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -225,6 +233,109 @@ def bench_batched(args, btk, torch, wl, dev):
     print(json.dumps(line), flush=True)
 
 
+def bench_forest(args, btk, torch, wl, dev):
+    """Many independent problems: all of them in one forest solve
+    (btk_solve_batch, FP64 and mixed) against the same problems solved one
+    by one with btk_solve (FP64)."""
+    dim, nprob, lo, hi, a, degree, theta, leaf, desc = FOREST[args.workload]
+    rng = np.random.default_rng(SEED + 11)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), nprob)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(off[-1])
+    pts = [wl.plummer_3d(int(m), SEED + 100 + i, a=a, rmax=10 * a) for i, m in enumerate(sizes)]
+    d = [torch.from_numpy(np.concatenate([p[k] for p in pts])).to(dev) for k in range(4)]
+    del pts
+    cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
+    kernel = btk.KERNEL_POT
+
+    def forest(prec, steps, warmup):
+        o = btk.options(kernel=kernel, exclude_self=1, precision=prec)
+        out = torch.zeros(n, dtype=torch.float64, device=dev)
+        for _ in range(warmup):
+            out.zero_()
+            btk.solve_batch(cfg, o, off, d[0], d[1], d[2], d[3], out)
+        torch.cuda.synchronize()
+        st = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out.zero_()
+            st.append(btk.solve_batch(cfg, o, off, d[0], d[1], d[2], d[3], out))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out, st
+
+    def one_by_one(steps, warmup):
+        o = btk.options(kernel=kernel, exclude_self=1)
+        out = torch.zeros(n, dtype=torch.float64, device=dev)
+
+        def run():
+            for p in range(nprob):
+                b, e = int(off[p]), int(off[p + 1])
+                btk.solve(cfg, o, d[0][b:e], d[1][b:e], d[2][b:e], d[3][b:e], out[b:e])
+        for _ in range(warmup):
+            out.zero_()
+            run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out.zero_()
+            run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out
+
+    t_m, out_m, st_m = forest(btk.PREC_MIXED, args.steps, args.warmup)
+    t_p, out_p, st_p = forest(btk.PREC_FP64, args.steps, args.warmup)
+    t_1, out_1 = one_by_one(max(1, args.steps // 2), 1)
+    # accuracy: 16 problems, up to 1000 sampled targets each, FP64 direct sum over the problem's sources
+    g = np.random.default_rng(SEED + 1)
+    num_m = num_p = num_1 = den = 0.0
+    for p in g.choice(nprob, 16, replace=False):
+        b, e = int(off[p]), int(off[p + 1])
+        m = e - b
+        idx = torch.from_numpy(np.sort(g.choice(m, min(m, 1000), replace=False)).astype(np.int32)).to(dev)
+        ds = torch.zeros(idx.numel(), dtype=torch.float64, device=dev)
+        src = [t[b:e] for t in d[:3]]
+        btk.direct_sum(3, btk.options(kernel=kernel, exclude_self=1), 0.0, *src, *src, d[3][b:e], ds, idx=idx)
+        sel = idx.long() + b
+        num_m += float(torch.sum((out_m[sel] - ds) ** 2))
+        num_p += float(torch.sum((out_p[sel] - ds) ** 2))
+        num_1 += float(torch.sum((out_1[sel] - ds) ** 2))
+        den += float(torch.sum(ds ** 2))
+    same = float(torch.linalg.norm(out_p - out_1) / torch.linalg.norm(out_1))
+
+    def avg(st, f):
+        return float(np.mean([getattr(s, f) for s in st]))
+    evals = [float(st_m[-1].evals[k]) for k in range(4)]
+    line = {
+        "metric": "targets/s at <=1e-6 rel-L2 vs FP64 direct sum; plain vs mixed/batched/learned",
+        "value": n / t_m,
+        "unit": "targets/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_m * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "name": args.workload, "problems": nprob, "n": n, "seed": SEED,
+                   "parallelism": "single GPU", "variant": "forest, mixed (FP32 kernel, FP64 accumulation)"},
+        "rel_l2": (num_m / den) ** 0.5, "rel_l2_plain_forest": (num_p / den) ** 0.5,
+        "rel_l2_one_by_one": (num_1 / den) ** 0.5, "rel_l2_targets": "16 problems x <=1000 targets",
+        "plain_forest_vs_one_by_one_rel": same,
+        "ms_plain_forest": t_p * 1e3, "ms_one_by_one_fp64": t_1 * 1e3,
+        "forest_fp64_over_one_by_one": t_1 / t_p, "forest_mixed_over_one_by_one": t_1 / t_m,
+        "phases_s": {k: avg(st_m, "t_" + k) for k in ("tree", "traversal", "upward", "interact", "downward",
+                                                      "total")},
+        "phases_plain_s": {k: avg(st_p, "t_" + k) for k in ("tree", "traversal", "upward", "interact",
+                                                            "downward", "total")},
+        "t_kernel_near_s": avg(st_m, "t_kernel_near"), "t_kernel_far_s": avg(st_m, "t_kernel_far"),
+        "evals": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
+        "panels": int(st_m[-1].panel_count_source), "levels": int(st_m[-1].levels_source),
+        "cpu_baseline": None,
+    }
+    t_near = avg(st_m, "t_kernel_near")
+    if t_near > 0:
+        ach = (evals[0] + evals[1]) * FLOPS[(3, 0)] / t_near / 1e12
+        line["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["f32"],
+                            "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS["f32"], 4), "traffic": None,
+                            "kernel": "k_interact near (PP+PC), FP32", "avg_launch_s": t_near}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -246,6 +357,8 @@ def main():
         return
     if args.workload in BATCHED:
         return bench_batched(args, btk, torch, wl, dev)
+    if args.workload in FOREST:
+        return bench_forest(args, btk, torch, wl, dev)
     spec = WORKLOADS[args.workload]
     dim, n, gen, degree, theta, leaf, kernel, desc = spec
     n = args.n or n

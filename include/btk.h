@@ -220,6 +220,17 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
                     const double *d_q, int nrhs, int coincident, double *d_out, btk_solve_stats *stats,
                     void *stream);
 
+/* Many independent problems in one solve (no reference counterpart): problem
+ * p owns points [offsets[p], offsets[p+1]) of d_x/d_y/d_z/d_q (targets =
+ * sources within a problem; offsets is a host array of nprob + 1 entries,
+ * offsets[0] = 0, every problem non-empty).  The problems form one forest --
+ * each problem's tree, list and result are those of its own btk_solve -- and
+ * every phase runs as one launch over all problems.  Adds into
+ * d_out[C][offsets[nprob]].  deepest_only is rejected when nprob > 1. */
+int btk_solve_batch(const btk_run_config *cfg, const btk_options *opts, int nprob, const int *offsets,
+                    const double *d_x, const double *d_y, const double *d_z, const double *d_q, double *d_out,
+                    btk_solve_stats *stats, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
