@@ -24,7 +24,9 @@ KERNEL_POT, KERNEL_REF_VEL3D, KERNEL_POT_GRAD = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTED = [
-    "btk_components", "btk_last_error", "btk_release_cached", "btk_read_run_config",
+    "btk_components", "btk_last_error", "btk_release_cached", "btk_set_tuning", "btk_get_tuning",
+    "btk_autotune",
+    "btk_read_run_config",
     "btk_tree_construction_2d", "btk_tree_construction_3d", "btk_tree_get_info",
     "btk_tree_copy_panels_2d", "btk_tree_copy_panels_3d", "btk_tree_copy_point_leaf_panel",
     "btk_tree_copy_panel_points_inside", "btk_tree_copy_permutation", "btk_tree_destroy",
@@ -57,6 +59,12 @@ class TreeInfo(C.Structure):
 class Options(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
                 ("deepest_only", C.c_int), ("precision", C.c_int), ("reserved", C.c_int * 3)]
+
+
+class Tuning(C.Structure):
+    """btk_tuning: launch shapes of the interaction kernels (0 = default)."""
+    _fields_ = [("near_items", C.c_int), ("far_items", C.c_int), ("near_tpt", C.c_int), ("far_tpt", C.c_int),
+                ("reserved", C.c_int * 4)]
 
 
 class SolveStats(C.Structure):
@@ -101,6 +109,10 @@ def lib():
     L.btk_components.argtypes = [C.c_int, C.c_int]
     L.btk_last_error.restype = C.c_char_p
     L.btk_read_run_config.argtypes = [C.c_char_p, cfgp]
+    L.btk_set_tuning.argtypes = [C.POINTER(Tuning)]
+    L.btk_get_tuning.argtypes = [C.POINTER(Tuning)]
+    L.btk_autotune.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int,
+                               C.POINTER(Tuning), vp]
     L.btk_tree_construction_2d.argtypes = [cfgp, dp, dp, C.c_int, vp, C.POINTER(vp)]
     L.btk_tree_construction_3d.argtypes = [cfgp, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
     L.btk_tree_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
@@ -142,6 +154,32 @@ def _check(rc: int) -> None:
 
 def components(dim: int, kernel: int) -> int:
     return lib().btk_components(dim, kernel)
+
+
+def get_tuning() -> Tuning:
+    t = Tuning()
+    _check(lib().btk_get_tuning(C.byref(t)))
+    return t
+
+
+def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0) -> None:
+    """Process-wide launch shapes of the interaction kernels; 0 = default."""
+    t = Tuning()
+    t.near_items, t.far_items, t.near_tpt, t.far_tpt = near_items, far_items, near_tpt, far_tpt
+    _check(lib().btk_set_tuning(C.byref(t)))
+
+
+def autotune(cfg, opts, xt, yt, zt, q, xs=None, ys=None, zs=None, stream=None) -> Tuning:
+    """Times the interaction launches of this problem under candidate shapes
+    and installs the fastest process-wide (btk_autotune)."""
+    coincident = xs is None
+    if coincident:
+        xs, ys, zs = xt, yt, zt
+    t = Tuning()
+    _check(lib().btk_autotune(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
+                              int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), int(coincident), C.byref(t),
+                              _stream(stream)))
+    return t
 
 
 def make_config(dim, degree, thresh, theta=0.7, eps=-1.0) -> RunConfig:

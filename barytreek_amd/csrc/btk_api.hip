@@ -22,16 +22,20 @@ thread_local std::string g_err;
 
 template <class F>
 int guarded(F &&f) {
+    int rc = 0;
     try {
         f();
-        return 0;
     } catch (const std::exception &e) {
         g_err = e.what();
-        return 1;
+        rc = 1;
     } catch (...) {
         g_err = "unknown error";
-        return 1;
+        rc = 1;
     }
+    // a failed call may leave work queued that reads blocks it freed
+    if (rc) (void)hipDeviceSynchronize();
+    btk::end_call();
+    return rc;
 }
 
 hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
@@ -102,6 +106,23 @@ const char *btk_last_error(void) { return g_err.c_str(); }
 
 int btk_release_cached(void) {
     return guarded([&] { btk::release_cached(); });
+}
+
+int btk_set_tuning(const btk_tuning *t) {
+    return guarded([&] {
+        BTK_REQUIRE(t, "null argument");
+        BTK_REQUIRE(t->near_items >= 0 && t->far_items >= 0, "work item targets must be >= 0");
+        BTK_REQUIRE(t->near_tpt >= 0 && t->near_tpt <= 2 && t->far_tpt >= 0 && t->far_tpt <= 2,
+                    "targets per lane must be 0 (default), 1 or 2");
+        btk::tuning() = *t;
+    });
+}
+
+int btk_get_tuning(btk_tuning *t) {
+    return guarded([&] {
+        BTK_REQUIRE(t, "null argument");
+        *t = btk::tuning();
+    });
 }
 
 // read_namelist.cpp:10-44.  The parse reads key=value lines and stops at the
@@ -489,6 +510,72 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
             stats->levels_target = tt.levels;
             stats->levels_source = ts->levels;
         }
+    });
+}
+
+int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
+                 const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                 const double *d_zs, const double *d_q, int coincident, btk_tuning *chosen, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in, "null run config");
+        btk_run_config cfg = *cfg_in;
+        const int dim = cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
+        const btk_options o = opts_or_default(opts);
+        hipStream_t s = as_stream(stream);
+        auto ts = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
+        std::unique_ptr<btk::Tree> tt_own;
+        if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
+        const btk::Tree &tt = coincident ? *ts : *tt_own;
+        auto l = btk::build_list(&cfg, tt, *ts, s);
+        const int P = btk::ipow(cfg.interp_degree + 1, dim);
+        const int C = btk::components(dim, o.kernel);
+        btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P), out((size_t)C * (m ? m : 1));
+        btk::gather_sorted(*ts, d_q, q.p, s);
+        btk::upward_pass(cfg, *ts, q.p, W.p, s);
+        // Candidate shapes: work-item targets x targets per lane (mixed
+        // precision only).  The near and far launches are independent, so
+        // candidate i tries near shape i and far shape i in the same call.
+        const long long items[] = {8192, 16384, 32768, 65536, 131072};
+        std::vector<std::pair<int, int>> cand;
+        for (int tpt = 1; tpt <= (o.precision == BTK_PREC_MIXED ? 2 : 1); tpt++)
+            for (long long it : items) cand.push_back({(int)it, tpt});
+        const btk_tuning saved = btk::tuning();
+        btk::KernelEvents kev;
+        for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
+        float best_near = 1e30f, best_far = 1e30f;
+        btk_tuning best{};
+        try {
+            for (int i = -1; i < (int)cand.size(); i++) {  // i = -1 warms up
+                const auto &c = cand[i < 0 ? 0 : i];
+                btk_tuning t{};
+                t.near_items = t.far_items = c.first;
+                t.near_tpt = t.far_tpt = c.second;
+                btk::tuning() = t;
+                kev.near_done = kev.far_done = false;
+                btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev);
+                BTK_HIP(hipStreamSynchronize(s));
+                if (i < 0) continue;
+                float ms = 0.f;
+                if (kev.near_done) {
+                    BTK_HIP(hipEventElapsedTime(&ms, kev.ev[0], kev.ev[1]));
+                    if (ms < best_near) best_near = ms, best.near_items = c.first, best.near_tpt = c.second;
+                }
+                if (kev.far_done) {
+                    BTK_HIP(hipEventElapsedTime(&ms, kev.ev[2], kev.ev[3]));
+                    if (ms < best_far) best_far = ms, best.far_items = c.first, best.far_tpt = c.second;
+                }
+            }
+        } catch (...) {
+            btk::tuning() = saved;
+            for (auto &e : kev.ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto &e : kev.ev) (void)hipEventDestroy(e);
+        if (o.precision != BTK_PREC_MIXED) best.near_tpt = best.far_tpt = 0;
+        btk::tuning() = best;
+        if (chosen) *chosen = best;
     });
 }
 

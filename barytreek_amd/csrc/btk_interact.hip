@@ -7,7 +7,10 @@ namespace {
 
 constexpr int TB = 256;
 constexpr int kMaxSplit = 16;
-constexpr long long kTargetWaves = 8192;  // 8 wavefronts per SIMD on 256 CUs
+// Default work-item targets: 64 (near) and 32 (far) wavefronts per SIMD of
+// 256 CUs; finer items balance the uneven leaf and cluster pairs.
+constexpr long long kNearItems = 65536;
+constexpr long long kFarItems = 32768;
 inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
 
 // Work items per target panel: blocks of `per` targets times entry slices.
@@ -19,6 +22,17 @@ __global__ void k_task_count(int pct, const int *__restrict__ goff, int far, con
     const int nt = far ? P : tcount[p];
     const int nb = (ne > 0 && nt > 0) ? (nt + per - 1) / per : 0;
     ntask[p] = nb * min(nsplit, ne);
+}
+
+// Source descriptor of every grouped entry (EntryView).
+__global__ void k_entry_meta(int n, const int *__restrict__ gsrc, const unsigned char *__restrict__ gtype,
+                             const int *__restrict__ start, const int *__restrict__ count, int P,
+                             int4 *__restrict__ meta) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const int S = gsrc[i];
+    const int ty = gtype[i];
+    meta[i] = (ty & 1) ? make_int4(S, 0, P, ty) : make_int4(S, start[S], count[S], ty);
 }
 
 // Entry slices balanced by source count (a leaf's points, or P proxies).
@@ -68,22 +82,26 @@ __global__ void k_task_fill(int pct, const int *__restrict__ goff, int far, cons
 
 template <int DIM, int KERNEL>
 void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt,
-               const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, double *dst, hipStream_t s,
-               KernelEvents *kev) {
+               const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, const int4 *meta, double *dst,
+               hipStream_t s, KernelEvents *kev) {
     const int n1 = cfg.interp_degree + 1;
     const int P = ipow(n1, DIM);
     const bool f32 = o.precision == BTK_PREC_MIXED;
-    const int tpt = (f32 && !(far && P <= kLanes)) ? 2 : 1;
+    const btk_tuning &tu = tuning();
+    const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
+    // default: two targets per lane where the kernel is short (2D), one in 3D
+    const int tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((DIM == 3 || (far && P <= kLanes)) ? 1 : 2));
     const int per = kLanes * tpt;
     const int n_clusters = tt.pc - tt.n_leaves;
     const long long base = far ? (long long)n_clusters * ((P + per - 1) / per)
                                : (long long)tt.n / per + tt.n_leaves;
-    const int nsplit = choose_split(base);
+    const long long items = far ? (tu.far_items ? tu.far_items : kFarItems) : (tu.near_items ? tu.near_items : kNearItems);
+    const int nsplit = choose_split(base, items);
     Tasks tk;
     build_tasks(l, tt, ts, far ? 1 : 0, P, per, nsplit, tk, s);
     InteractLaunch L;
     L.tk = TaskView{tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p};
-    L.gv = GroupView{l.gsrc.p, l.gtype.p};
+    L.ev = EntryView{meta};
     L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
                    tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
     L.sv = sv;
@@ -112,8 +130,19 @@ template <int DIM, int KERNEL>
 void run_all(const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt, const Tree &ts,
              const List &l, const SrcView &sv, const NodeTables &tabs, double *out, double *Psi, hipStream_t s,
              KernelEvents *kev) {
-    if (l.counts[0] + l.counts[1] > 0) run_group<DIM, KERNEL>(false, cfg, o, kp, tt, ts, l, sv, tabs, out, s, kev);
-    if (l.counts[2] + l.counts[3] > 0) run_group<DIM, KERNEL>(true, cfg, o, kp, tt, ts, l, sv, tabs, Psi, s, kev);
+    DBuf<int4> meta(l.count ? l.count : 1);
+    if (l.count) {
+        k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p,
+                                                  ipow(cfg.interp_degree + 1, DIM), meta.p);
+        BTK_LAUNCH_CHECK();
+    }
+    if (l.counts[0] + l.counts[1] > 0)
+        run_group<DIM, KERNEL>(false, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, out, s, kev);
+    if (l.counts[2] + l.counts[3] > 0)
+        run_group<DIM, KERNEL>(true, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, Psi, s, kev);
+    // meta goes back to the cache on return, while the launches may still run;
+    // the cache hands it out only to this thread's later work on this stream
+    // until the API call has synchronized
 }
 
 // ---------------------------------------------------------------------------
@@ -191,9 +220,14 @@ void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, 
     BTK_LAUNCH_CHECK();
 }
 
-int choose_split(long long base_items) {
+btk_tuning &tuning() {
+    static btk_tuning t{};
+    return t;
+}
+
+int choose_split(long long base_items, long long target_items) {
     if (base_items <= 0) return 1;
-    long long k = (kTargetWaves + base_items - 1) / base_items;
+    long long k = (target_items + base_items - 1) / base_items;
     return (int)std::max(1ll, std::min<long long>(kMaxSplit, k));
 }
 

@@ -24,6 +24,8 @@
 // sums over one 64-source tile, added into FP64 per-target accumulators.
 #pragma once
 
+#include <type_traits>
+
 #include "btk_device.hpp"
 
 namespace btk {
@@ -52,8 +54,29 @@ struct GroupView {
     const unsigned char *type;
 };
 
-struct Src64 {
-    double x, y, z, q;
+// One source in the LDS tile.  The charge (or proxy weight) is rounded to
+// FP32 once, when the tile is filled, in the mixed-precision kernels.
+template <bool F32>
+struct TileSrc {
+    double x, y, z;
+    std::conditional_t<F32, float, double> q;
+};
+
+// A source of the far (CP, CC) kernels in mixed precision: coordinates
+// relative to the target cluster's box centre, rounded to FP32 once.  Far
+// pairs satisfy the MAC, (r_t + r_s) / |c_s - c_t| < theta, so |t - c_t| +
+// |s - c_t| is at most (1 + theta) / (1 - theta) times |t - s|, and the FP32
+// rounding of the relative coordinates perturbs t - s by at most that factor
+// times 2^-24 (relative): 3.4e-7 at theta = 0.7.
+struct TileRel {
+    float x, y, z, q;
+};
+
+// Per-entry source descriptor of a grouped list: x = source panel, y = first
+// sorted point of its points (0 for proxy sources), z = number of sources
+// (the leaf's point count, or P proxy points), w = interact_type.
+struct EntryView {
+    const int4 *meta;
 };
 
 __device__ __forceinline__ double sel3(int d, double a, double b, double c) { return d == 0 ? a : (d == 1 ? b : c); }
@@ -77,12 +100,54 @@ __device__ __forceinline__ void proxy_point(int idx, int n1, const NodeTables &t
 }
 
 template <int DIM, int KERNEL, bool F32, bool EXCL>
-__device__ __forceinline__ void eval_pair(const Src64 &s, const double *t, double eps2, double *acc64, float *acc32) {
+__device__ __forceinline__ void eval_pair(const TileSrc<F32> &s, const double *t, double eps2, double *acc64,
+                                          float *acc32) {
     const double dx = t[0] - s.x, dy = t[1] - s.y, dz = DIM == 3 ? t[2] - s.z : 0.0;
     if constexpr (F32)
-        eval_f32<DIM, KERNEL, EXCL>((float)dx, (float)dy, (float)dz, (float)s.q, (float)eps2, acc32);
+        eval_f32<DIM, KERNEL, EXCL>((float)dx, (float)dy, (float)dz, s.q, (float)eps2, acc32);
     else
         eval_f64<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc64);
+}
+
+// Source j of entry m: a point of a source leaf, or a proxy point of a
+// source cluster with its weight.
+template <int DIM, bool F32>
+__device__ __forceinline__ TileSrc<F32> load_source(const int4 &m, int j, int n1, const NodeTables &tabs,
+                                                   const SrcView &sv) {
+    TileSrc<F32> s{0.0, 0.0, 0.0, 0};
+    if (j >= m.z) return s;
+    const int S = m.x;
+    if (m.w & 1) {
+        double p[3];
+        proxy_point<DIM>(j, n1, tabs, sv.lo0[S], sv.hi0[S], sv.lo1[S], sv.hi1[S], DIM == 3 ? sv.lo2[S] : 0.0,
+                         DIM == 3 ? sv.hi2[S] : 0.0, p);
+        const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
+        s.x = p[0];
+        s.y = p[1];
+        s.z = p[2];
+        s.q = (decltype(s.q))sv.W[(size_t)S * P + j];
+    } else {
+        const int i = m.y + j;
+        s.x = sv.sx[i];
+        s.y = sv.sy[i];
+        if constexpr (DIM == 3) s.z = sv.sz[i];
+        s.q = (decltype(s.q))sv.q[i];
+    }
+    return s;
+}
+
+// The same source relative to the centre c, in FP32.
+template <int DIM>
+__device__ __forceinline__ TileRel load_source_rel(const int4 &m, int j, int n1, const NodeTables &tabs,
+                                                   const SrcView &sv, const double *c) {
+    const TileSrc<true> a = load_source<DIM, true>(m, j, n1, tabs, sv);
+    if (j >= m.z) return TileRel{0.0f, 0.0f, 0.0f, 0.0f};
+    return TileRel{(float)(a.x - c[0]), (float)(a.y - c[1]), DIM == 3 ? (float)(a.z - c[2]) : 0.0f, a.q};
+}
+
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_pair_rel(const TileRel &s, const float *t, float eps2, float *acc32) {
+    eval_f32<DIM, KERNEL, EXCL>(t[0] - s.x, t[1] - s.y, DIM == 3 ? t[2] - s.z : 0.0f, s.q, eps2, acc32);
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin.  Work items are
@@ -102,11 +167,12 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
 
 template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL>
 __global__ void __launch_bounds__(kLanes)
-    k_interact(TaskView tk, GroupView gv, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
+    k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
-    __shared__ Src64 tile[kLanes];
-    __shared__ double cn[3][kMaxN1];
+    constexpr bool REL = F32 && FAR;  // relative FP32 coordinates (TileRel)
+    using Tile = std::conditional_t<REL, TileRel, TileSrc<F32>>;
+    __shared__ Tile tile[kLanes];
     const int lane = threadIdx.x;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
     const int T = tk.panel[item];
@@ -116,7 +182,14 @@ __global__ void __launch_bounds__(kLanes)
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
     double t[TPT][3];
+    float tr[TPT][3];
     double acc[TPT][C];
+    double ctr[3] = {0.0, 0.0, 0.0};
+    if constexpr (REL) {
+        ctr[0] = 0.5 * (tv.lo0[T] + tv.hi0[T]);
+        ctr[1] = 0.5 * (tv.lo1[T] + tv.hi1[T]);
+        if constexpr (DIM == 3) ctr[2] = 0.5 * (tv.lo2[T] + tv.hi2[T]);
+    }
 #pragma unroll
     for (int k = 0; k < TPT; k++) {
         const int ti = min(first + lane + kLanes * k, ntgt - 1);
@@ -130,81 +203,84 @@ __global__ void __launch_bounds__(kLanes)
             t[k][2] = DIM == 3 ? tv.sz[ts0 + ti] : 0.0;
         }
 #pragma unroll
+        for (int d = 0; d < 3; d++) tr[k][d] = (float)(t[k][d] - ctr[d]);
+#pragma unroll
         for (int c = 0; c < C; c++) acc[k][c] = 0.0;
     }
-    for (int e = e0; e < e1; e++) {
-        const int S = gv.src[e];
-        const int ty = gv.type[e];
-        const bool cluster = (ty & 1) != 0;  // PC, CC: sources are the proxy points of S
-        const int nsrc = cluster ? P : sv.count[S];
-        if (cluster) {
-            if (lane < DIM * n1) {
-                const int d = lane / n1, b = lane % n1;
-                cn[d][b] = proxy_node(tabs.b[b], sel3(d, sv.lo0[S], sv.lo1[S], sv.lo2[S]),
-                                      sel3(d, sv.hi0[S], sv.hi1[S], sv.hi2[S]));
-            }
-            __syncthreads();
+    auto fetch = [&](const int4 &mm, int j) -> Tile {
+        if constexpr (REL)
+            return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
+        else
+            return load_source<DIM, F32>(mm, j, n1, tabs, sv);
+    };
+    auto eval = [&](const Tile &s, int k, float *a32, auto excl) {
+        constexpr bool X = decltype(excl)::value;
+        if constexpr (REL)
+            eval_pair_rel<DIM, KERNEL, X>(s, tr[k], (float)kp.eps2, a32);
+        else
+            eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
+    };
+    // The entries' sources stream through the tile 64 at a time, as one
+    // sequence over all entries of the item; the next tile's loads are
+    // issued before the current tile is evaluated.
+    int e = e0, j0 = 0;
+    int4 m = make_int4(0, 0, 0, 0);
+    while (e < e1 && (m = ev.meta[e]).z == 0) e++;
+    Tile next{};
+    if (e < e1) next = fetch(m, lane);
+    while (e < e1) {
+        tile[lane] = next;
+        __syncthreads();
+        const int nt = min(kLanes, m.z - j0);
+        // With one tree for targets and sources, coincident points share a
+        // leaf, so among point sources only the target leaf's own entry can
+        // hold a zero-distance pair.  Proxy sources keep the test.
+        const bool check = !same_tree || m.x == T || (m.w & 1);
+        int en = e, jn = j0 + kLanes;
+        int4 mn = m;
+        if (jn >= m.z) {
+            jn = 0;
+            en++;
+            while (en < e1 && (mn = ev.meta[en]).z == 0) en++;
         }
-        // With one tree for targets and sources, coincident points share a leaf,
-        // so among point sources only the target leaf's own entry can hold a
-        // zero-distance pair.  Proxy sources keep the test.
-        const bool check = !same_tree || S == T || cluster;
-        const int ss = cluster ? 0 : sv.start[S];
-        const double *w = sv.W + (size_t)S * P;
-        for (int t0 = 0; t0 < nsrc; t0 += kLanes) {
-            const int j = t0 + lane;
-            if (j < nsrc) {
-                Src64 s;
-                if (cluster) {
-                    if constexpr (DIM == 3) {
-                        const int l = j % n1, k = (j / n1) % n1, jj = j / (n1 * n1);
-                        s = Src64{cn[0][jj], cn[1][k], cn[2][l], w[j]};
-                    } else {
-                        const int k = j % n1, jj = j / n1;
-                        s = Src64{cn[0][jj], cn[1][k], 0.0, w[j]};
-                    }
-                } else {
-                    s = Src64{sv.sx[ss + j], sv.sy[ss + j], DIM == 3 ? sv.sz[ss + j] : 0.0, sv.q[ss + j]};
-                }
-                tile[lane] = s;
+        if (en < e1) next = fetch(mn, jn + lane);
+        float a32[TPT][C];
+#pragma unroll
+        for (int k = 0; k < TPT; k++)
+#pragma unroll
+            for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
+        if (nt == kLanes && EXCL && !check) {
+            // no zero-distance pair is possible here: skip the exclusion test
+#pragma unroll 8
+            for (int jj = 0; jj < kLanes; jj++) {
+                const Tile s = tile[jj];
+#pragma unroll
+                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{});
             }
-            __syncthreads();
-            const int nt = min(kLanes, nsrc - t0);
-            float a32[TPT][C];
+        } else if (nt == kLanes) {
+#pragma unroll 8
+            for (int jj = 0; jj < kLanes; jj++) {
+                const Tile s = tile[jj];
+#pragma unroll
+                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{});
+            }
+        } else {
+            for (int jj = 0; jj < nt; jj++) {
+                const Tile s = tile[jj];
+#pragma unroll
+                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{});
+            }
+        }
+        if constexpr (F32) {
 #pragma unroll
             for (int k = 0; k < TPT; k++)
 #pragma unroll
-                for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
-            if (nt == kLanes && EXCL && !check) {
-                // no zero-distance pair is possible here: skip the exclusion test
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Src64 s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval_pair<DIM, KERNEL, F32, false>(s, t[k], kp.eps2, acc[k], a32[k]);
-                }
-            } else if (nt == kLanes) {
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Src64 s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval_pair<DIM, KERNEL, F32, EXCL>(s, t[k], kp.eps2, acc[k], a32[k]);
-                }
-            } else {
-                for (int jj = 0; jj < nt; jj++) {
-                    const Src64 s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval_pair<DIM, KERNEL, F32, EXCL>(s, t[k], kp.eps2, acc[k], a32[k]);
-                }
-            }
-            if constexpr (F32) {
-#pragma unroll
-                for (int k = 0; k < TPT; k++)
-#pragma unroll
-                    for (int c = 0; c < C; c++) acc[k][c] += (double)a32[k][c];
-            }
-            __syncthreads();
+                for (int c = 0; c < C; c++) acc[k][c] += (double)a32[k][c];
         }
+        __syncthreads();
+        e = en;
+        j0 = jn;
+        m = mn;
     }
 #pragma unroll
     for (int k = 0; k < TPT; k++) {
@@ -234,13 +310,14 @@ struct Tasks {
 };
 void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
                  hipStream_t s);
-// Entry slices per panel so that about 8 wavefronts per SIMD are in flight.
-int choose_split(long long base_items);
+// Entry slices per panel so that the launch has about target_items work
+// items (at most 16 slices per panel).
+int choose_split(long long base_items, long long target_items);
 
 // Launch of one interaction group with everything fixed but the kernel.
 struct InteractLaunch {
     TaskView tk;
-    GroupView gv;
+    EntryView ev;
     TgtView tv;
     SrcView sv;
     int n_tasks, n1, tpt, far, excl, f32, atomic;
@@ -259,7 +336,7 @@ void launch_interact(const InteractLaunch &L, hipStream_t s);
 namespace btk {
 
 #define BTK_IK(F32_, TPT_, FAR_, EXCL_)                                                                          \
-    k_interact<DIM, KERNEL, F32_, TPT_, FAR_, EXCL_><<<L.n_tasks, kLanes, 0, s>>>(L.tk, L.gv, L.tv, L.sv, L.n1,  \
+    k_interact<DIM, KERNEL, F32_, TPT_, FAR_, EXCL_><<<L.n_tasks, kLanes, 0, s>>>(L.tk, L.ev, L.tv, L.sv, L.n1,  \
                                                                                  L.tabs, L.kp, L.out,            \
                                                                                  L.comp_stride, L.atomic, L.same_tree)
 #define BTK_IK_FE(F32_, TPT_)                                  \

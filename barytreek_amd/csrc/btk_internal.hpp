@@ -40,6 +40,8 @@ struct Error : std::runtime_error {
 void *dmalloc(size_t bytes);
 void dfree(void *p);
 void release_cached();
+// Called when an API call ends, after its stream has been synchronized.
+void end_call();
 
 template <class T>
 struct DBuf {
@@ -164,6 +166,9 @@ struct KParams {
 };
 KParams make_kparams(int dim, double eps, const btk_options &o);
 int components(int dim, int kernel);
+
+// Process-wide launch shapes of the interaction kernels (zero = default).
+btk_tuning &tuning();
 
 // Phases.
 std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const double *y, const double *z, int n,

@@ -268,10 +268,10 @@ void run_multi(const btk_run_config &cfg, const btk_options &o, const Tree &tt, 
     for (int far = 0; far < 2; far++) {
         if (l.counts[2 * far] + l.counts[2 * far + 1] == 0) continue;
         const int n_clusters = tt.pc - tt.n_leaves;
-        // workgroups of 4 waves: aim at 2048 of them
+        // base counts wavefronts (workgroups of 4): aim at 8192 of them
         const long long base = (far ? (long long)n_clusters * ((P + 63) / 64) : (long long)tt.n / 64 + tt.n_leaves) *
                                chunks * 4;
-        const int nsplit = choose_split(base);
+        const int nsplit = choose_split(base, 8192);
         Tasks tk;
         build_tasks(l, tt, ts, far, P, 64, nsplit, tk, s);
         if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));

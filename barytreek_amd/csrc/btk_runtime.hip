@@ -3,10 +3,12 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <numbers>
 #include <unordered_map>
+#include <vector>
 
 #include "btk_internal.hpp"
 
@@ -16,16 +18,31 @@ namespace btk {
 // Cached device allocator.  Blocks are kept by size and handed out again to
 // requests between 1x and 2x of their size; nothing is returned to HIP until
 // release_cached().  Allocations are per current device.
+//
+// A block freed during an API call may still be read by work queued on that
+// call's stream, so until the call ends (every entry point synchronizes its
+// stream before it returns) the block is handed out only to the same thread,
+// whose later work is queued behind it on the same stream.  end_call() makes
+// the thread's freed blocks available to every thread.
 // ---------------------------------------------------------------------------
 namespace {
 struct Block {
     size_t bytes;
     int device;
+    int owner;  // thread that freed it during a call that has not ended; 0 = any
 };
 std::mutex g_mu;
 std::multimap<std::pair<int, size_t>, void *> g_free;  // (device, bytes) -> ptr
 std::unordered_map<void *, Block> g_used;
 std::unordered_map<void *, Block> g_cached;
+std::atomic<int> g_next_thread{1};
+thread_local int t_thread = 0;
+thread_local std::vector<void *> t_freed;
+
+int this_thread() {
+    if (!t_thread) t_thread = g_next_thread.fetch_add(1);
+    return t_thread;
+}
 
 size_t round_bytes(size_t b) {
     if (b < 256) return 256;
@@ -37,15 +54,17 @@ void *dmalloc(size_t bytes) {
     int dev = 0;
     BTK_HIP(hipGetDevice(&dev));
     size_t b = round_bytes(bytes);
+    const int me = this_thread();
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_free.lower_bound({dev, b});
-        if (it != g_free.end() && it->first.first == dev && it->first.second <= 2 * b) {
+        for (auto it = g_free.lower_bound({dev, b});
+             it != g_free.end() && it->first.first == dev && it->first.second <= 2 * b; ++it) {
             void *p = it->second;
-            Block blk{it->first.second, dev};
+            const Block blk = g_cached[p];
+            if (blk.owner != 0 && blk.owner != me) continue;
             g_free.erase(it);
             g_cached.erase(p);
-            g_used[p] = blk;
+            g_used[p] = Block{blk.bytes, dev, 0};
             return p;
         }
     }
@@ -57,7 +76,7 @@ void *dmalloc(size_t bytes) {
         BTK_HIP(hipMalloc(&p, b));
     }
     std::lock_guard<std::mutex> lk(g_mu);
-    g_used[p] = Block{b, dev};
+    g_used[p] = Block{b, dev, 0};
     return p;
 }
 
@@ -68,8 +87,19 @@ void dfree(void *p) {
     if (it == g_used.end()) return;
     Block blk = it->second;
     g_used.erase(it);
+    blk.owner = this_thread();
     g_free.emplace(std::make_pair(blk.device, blk.bytes), p);
     g_cached[p] = blk;
+    t_freed.push_back(p);
+}
+
+void end_call() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (void *p : t_freed) {
+        auto it = g_cached.find(p);
+        if (it != g_cached.end() && it->second.owner == t_thread) it->second.owner = 0;
+    }
+    t_freed.clear();
 }
 
 void release_cached() {

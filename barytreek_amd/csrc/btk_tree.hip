@@ -507,9 +507,10 @@ std::unique_ptr<Tree> build_forest(int dim, int thresh, const double *x, const d
         k_iota<<<nblk(n), TB, 0, s>>>(n, t->perm.p, t->point_leaf.p);
         BTK_LAUNCH_CHECK();
     }
-    // root boxes: chunks of at most 64k points per problem, then one block per problem
+    // root boxes: chunks of at most 2048 points per problem (one workgroup
+    // each, 8 points per thread), then one workgroup per problem
     std::vector<int> cb, ce, pc0(1, 0);
-    const int chunk = std::max(65536, n / 1024 + 1);
+    const int chunk = 2048;
     for (int p = 0; p < nprob; p++) {
         int b = offsets[p];
         do {

@@ -73,10 +73,16 @@ def parse():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scale for the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def log(msg):
+    """Progress on stderr (the JSON line is the only stdout output)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(wl_spec, x, y, z, q, stride):
@@ -384,6 +390,7 @@ def main():
             idx = torch.from_numpy(np.sort(g.choice(n, 10_000, replace=False)).astype(np.int32)).to(dev)
         m = n if idx is None else int(idx.numel())
         ds = torch.zeros(C * m, dtype=torch.float64, device=dev)
+        log(f"FP64 direct sum on {m} targets")
         btk.direct_sum(dim, btk.options(kernel=kernel, convention=1, exclude_self=1), 0.0, d[0], d[1], d[2],
                        d[0], d[1], d[2], d[3], ds, idx=idx)
         torch.cuda.synchronize()
@@ -393,8 +400,16 @@ def main():
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
         opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v])
+        btk.set_tuning()  # defaults
+        if not args.no_autotune:  # untimed, like the warmup
+            log(f"autotune {v}")
+            btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
+        t = btk.get_tuning()
+        log(f"timing {v}")
         ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
         res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
+        res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
+                                   "far_tpt": t.far_tpt, "autotuned": not args.no_autotune}
         del out
     head = "mixed" if "mixed" in res else variants[0]
     main_r = res[head]
@@ -403,6 +418,7 @@ def main():
     at = None
     if ds is not None and not args.no_degree_scan:
         for deg in range(degree, 11):
+            log(f"degree scan: degree {deg}")
             cfg = btk.make_config(dim, deg, leaf, theta, 0.0)
             opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[head])
             ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, 1)
@@ -447,6 +463,7 @@ def main():
     if "plain" in res and "mixed" in res:
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
     if not args.no_cpu_baseline:
+        log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, args.cpu_stride)
         line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)

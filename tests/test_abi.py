@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import barytreek_amd as btk
 from barytreek_amd import PANEL2D_DTYPE, PANEL3D_DTYPE, RunConfig, Options, SolveStats
 
@@ -34,6 +36,22 @@ def test_record_layouts_match_reference():
     assert PANEL2D_DTYPE.itemsize == 80 and PANEL3D_DTYPE.itemsize == 112
     assert ctypes.sizeof(Options) == 32
     assert ctypes.sizeof(SolveStats) == 6 * 8 + 4 * 8 + 4 * 8 + 2 * 8 + 4 * 4
+    assert ctypes.sizeof(btk.Tuning) == 32
+
+
+def test_tuning_round_trip_and_rejects_bad_values():
+    """Host-only: the launch-shape record is stored and validated without a GPU."""
+    old = btk.get_tuning()
+    try:
+        btk.set_tuning(near_items=32768, far_tpt=1)
+        t = btk.get_tuning()
+        assert (t.near_items, t.far_items, t.near_tpt, t.far_tpt) == (32768, 0, 0, 1)
+        with pytest.raises(btk.BtkError, match="targets per lane"):
+            btk.set_tuning(near_tpt=3)
+        with pytest.raises(btk.BtkError, match="work item"):
+            btk.set_tuning(far_items=-1)
+    finally:
+        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt)
 
 
 def test_components():
