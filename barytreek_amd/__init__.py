@@ -64,7 +64,7 @@ class Options(C.Structure):
 class Tuning(C.Structure):
     """btk_tuning: launch shapes of the interaction kernels (0 = default)."""
     _fields_ = [("near_items", C.c_int), ("far_items", C.c_int), ("near_tpt", C.c_int), ("far_tpt", C.c_int),
-                ("reserved", C.c_int * 4)]
+                ("mutual", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class SolveStats(C.Structure):
@@ -162,10 +162,11 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0) -> None:
+def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0) -> None:
     """Process-wide launch shapes of the interaction kernels; 0 = default."""
     t = Tuning()
     t.near_items, t.far_items, t.near_tpt, t.far_tpt = near_items, far_items, near_tpt, far_tpt
+    t.mutual = mutual
     _check(lib().btk_set_tuning(C.byref(t)))
 
 

@@ -31,6 +31,7 @@ SOURCES = [
     "btk_ik_3d_vel.hip",
     "btk_ik_3d_grad.hip",
     "btk_mrhs.hip",
+    "btk_mutual.hip",
     "btk_api.hip",
 ]
 

@@ -114,6 +114,7 @@ int btk_set_tuning(const btk_tuning *t) {
         BTK_REQUIRE(t->near_items >= 0 && t->far_items >= 0, "work item targets must be >= 0");
         BTK_REQUIRE(t->near_tpt >= 0 && t->near_tpt <= 2 && t->far_tpt >= 0 && t->far_tpt <= 2,
                     "targets per lane must be 0 (default), 1 or 2");
+        BTK_REQUIRE(t->mutual == 0 || t->mutual == 1, "mutual must be 0 or 1");
         btk::tuning() = *t;
     });
 }
@@ -534,37 +535,58 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
         btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P), out((size_t)C * (m ? m : 1));
         btk::gather_sorted(*ts, d_q, q.p, s);
         btk::upward_pass(cfg, *ts, q.p, W.p, s);
-        // Candidate shapes: work-item targets x targets per lane (mixed
-        // precision only).  The near and far launches are independent, so
-        // candidate i tries near shape i and far shape i in the same call.
+        // Candidate shapes: work-item targets x targets per lane x mutual
+        // (the last two in mixed precision only).  The near and far launches
+        // are independent, so call i tries near shape i and far shape i.
+        const bool mixed = o.precision == BTK_PREC_MIXED;
         const long long items[] = {8192, 16384, 32768, 65536, 131072};
-        std::vector<std::pair<int, int>> cand;
-        for (int tpt = 1; tpt <= (o.precision == BTK_PREC_MIXED ? 2 : 1); tpt++)
-            for (long long it : items) cand.push_back({(int)it, tpt});
+        struct Shape {
+            int items, tpt, mutual;
+        };
+        std::vector<Shape> nc, fc;
+        for (int mu = 0; mu <= (mixed && coincident ? 1 : 0); mu++)
+            for (int tpt = 1; tpt <= (mixed ? 2 : 1); tpt++)
+                for (long long it : items) nc.push_back({(int)it, tpt, mu});
+        for (int tpt = 1; tpt <= (mixed ? 2 : 1); tpt++)
+            for (long long it : items) fc.push_back({(int)it, tpt, 0});
+        const int ncall = (int)std::max(nc.size(), fc.size());
         const btk_tuning saved = btk::tuning();
         btk::KernelEvents kev;
         for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
         float best_near = 1e30f, best_far = 1e30f;
         btk_tuning best{};
         try {
-            for (int i = -1; i < (int)cand.size(); i++) {  // i = -1 warms up
-                const auto &c = cand[i < 0 ? 0 : i];
+            for (int i = -1; i < ncall; i++) {  // i = -1 warms up
+                const Shape &a = nc[std::min<int>(std::max(i, 0), (int)nc.size() - 1)];
+                const Shape &b = fc[std::min<int>(std::max(i, 0), (int)fc.size() - 1)];
                 btk_tuning t{};
-                t.near_items = t.far_items = c.first;
-                t.near_tpt = t.far_tpt = c.second;
+                t.near_items = a.items;
+                t.near_tpt = a.tpt;
+                t.mutual = a.mutual;
+                t.far_items = b.items;
+                t.far_tpt = b.tpt;
                 btk::tuning() = t;
                 kev.near_done = kev.far_done = false;
                 btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev);
                 BTK_HIP(hipStreamSynchronize(s));
                 if (i < 0) continue;
                 float ms = 0.f;
-                if (kev.near_done) {
+                if (kev.near_done && i < (int)nc.size()) {
                     BTK_HIP(hipEventElapsedTime(&ms, kev.ev[0], kev.ev[1]));
-                    if (ms < best_near) best_near = ms, best.near_items = c.first, best.near_tpt = c.second;
+                    if (ms < best_near) {
+                        best_near = ms;
+                        best.near_items = a.items;
+                        best.near_tpt = a.tpt;
+                        best.mutual = a.mutual;
+                    }
                 }
-                if (kev.far_done) {
+                if (kev.far_done && i < (int)fc.size()) {
                     BTK_HIP(hipEventElapsedTime(&ms, kev.ev[2], kev.ev[3]));
-                    if (ms < best_far) best_far = ms, best.far_items = c.first, best.far_tpt = c.second;
+                    if (ms < best_far) {
+                        best_far = ms;
+                        best.far_items = b.items;
+                        best.far_tpt = b.tpt;
+                    }
                 }
             }
         } catch (...) {

@@ -14,11 +14,19 @@ constexpr long long kFarItems = 32768;
 inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
 
 // Work items per target panel: blocks of `per` targets times entry slices.
+// With meta, entries whose source count is zero (taken by another kernel)
+// are not counted, and a panel left without entries gets no work item.
 __global__ void k_task_count(int pct, const int *__restrict__ goff, int far, const int *__restrict__ tcount, int P,
-                             int per, int nsplit, int *__restrict__ ntask) {
+                             int per, int nsplit, const int4 *__restrict__ meta, int *__restrict__ ntask) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= pct) return;
-    const int ne = goff[far * pct + p + 1] - goff[far * pct + p];
+    const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
+    int ne = g1 - g0;
+    if (meta) {
+        int nz = 0;
+        for (int e = g0; e < g1; e++) nz += meta[e].z > 0;
+        ne = nz;
+    }
     const int nt = far ? P : tcount[p];
     const int nb = (ne > 0 && nt > 0) ? (nt + per - 1) / per : 0;
     ntask[p] = nb * min(nsplit, ne);
@@ -38,23 +46,29 @@ __global__ void k_entry_meta(int n, const int *__restrict__ gsrc, const unsigned
 // Entry slices balanced by source count (a leaf's points, or P proxies).
 __global__ void k_task_fill(int pct, const int *__restrict__ goff, int far, const int *__restrict__ gsrc,
                             const unsigned char *__restrict__ gtype, const int *__restrict__ scount, int P, int per,
-                            int nsplit, const int *__restrict__ tcount, const int *__restrict__ ntask,
-                            const int *__restrict__ tbase, int *__restrict__ tpanel, int *__restrict__ tfirst,
-                            int *__restrict__ te0, int *__restrict__ te1) {
+                            int nsplit, const int4 *__restrict__ meta, const int *__restrict__ tcount,
+                            const int *__restrict__ ntask, const int *__restrict__ tbase, int *__restrict__ tpanel,
+                            int *__restrict__ tfirst, int *__restrict__ te0, int *__restrict__ te1) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= pct || ntask[p] == 0) return;
     const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
-    const int ne = g1 - g0;
+    auto nsrc = [&](int e) -> long long { return meta ? meta[e].z : ((gtype[e] & 1) ? P : scount[gsrc[e]]); };
+    int ne = g1 - g0;
+    if (meta) {
+        int nz = 0;
+        for (int e = g0; e < g1; e++) nz += meta[e].z > 0;
+        ne = nz;
+    }
     const int ns = min(nsplit, ne);
     const int nt = far ? P : tcount[p];
     const int nb = (nt + per - 1) / per;
     long long total = 0;
-    for (int e = g0; e < g1; e++) total += (gtype[e] & 1) ? P : scount[gsrc[e]];
+    for (int e = g0; e < g1; e++) total += nsrc(e);
     const int base = tbase[p];
     int k = 0, start = g0;
     long long cum = 0;
     for (int e = g0; e < g1 && k < ns - 1; e++) {
-        cum += (gtype[e] & 1) ? P : scount[gsrc[e]];
+        cum += nsrc(e);
         // close slice k once it holds its share, leaving one entry for each later slice
         if (cum * ns >= (k + 1) * total && (g1 - (e + 1)) >= (ns - 1 - k)) {
             for (int b = 0; b < nb; b++) {
@@ -82,8 +96,8 @@ __global__ void k_task_fill(int pct, const int *__restrict__ goff, int far, cons
 
 template <int DIM, int KERNEL>
 void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt,
-               const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, const int4 *meta, double *dst,
-               hipStream_t s, KernelEvents *kev) {
+               const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, int4 *meta, double *dst,
+               const double *q_sorted, hipStream_t s, KernelEvents *kev) {
     const int n1 = cfg.interp_degree + 1;
     const int P = ipow(n1, DIM);
     const bool f32 = o.precision == BTK_PREC_MIXED;
@@ -97,8 +111,12 @@ void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const 
                                : (long long)tt.n / per + tt.n_leaves;
     const long long items = far ? (tu.far_items ? tu.far_items : kFarItems) : (tu.near_items ? tu.near_items : kNearItems);
     const int nsplit = choose_split(base, items);
+    // mixed precision, one tree, when asked for: mutual PP pairs once for both directions
+    Mutual mu;
+    const bool mutual = !far && f32 && &tt == &ts && tu.mutual;
+    if (mutual) build_mutual(l, tt, tpt, meta, mu, s);
     Tasks tk;
-    build_tasks(l, tt, ts, far ? 1 : 0, P, per, nsplit, tk, s);
+    build_tasks(l, tt, ts, far ? 1 : 0, P, per, nsplit, tk, s, mutual ? meta : nullptr);
     InteractLaunch L;
     L.tk = TaskView{tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p};
     L.ev = EntryView{meta};
@@ -119,6 +137,7 @@ void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const 
     L.out = dst;
     L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
     if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
+    if (mutual) launch_mutual<DIM, KERNEL>(mu, tt, q_sorted, kp, dst, s);
     launch_interact<DIM, KERNEL>(L, s);
     if (kev) {
         BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
@@ -137,9 +156,9 @@ void run_all(const btk_run_config &cfg, const btk_options &o, const KParams &kp,
         BTK_LAUNCH_CHECK();
     }
     if (l.counts[0] + l.counts[1] > 0)
-        run_group<DIM, KERNEL>(false, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, out, s, kev);
+        run_group<DIM, KERNEL>(false, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, out, sv.q, s, kev);
     if (l.counts[2] + l.counts[3] > 0)
-        run_group<DIM, KERNEL>(true, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, Psi, s, kev);
+        run_group<DIM, KERNEL>(true, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, Psi, sv.q, s, kev);
     // meta goes back to the cache on return, while the launches may still run;
     // the cache hands it out only to this thread's later work on this stream
     // until the API call has synchronized
@@ -199,10 +218,10 @@ void launch_direct(const KParams &kp, int m, const int *idx, const double *xt, c
 }  // namespace
 
 void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
-                 hipStream_t s) {
+                 hipStream_t s, const int4 *meta) {
     const int pct = tt.pc;
     DBuf<int> nt(pct), tb(pct);
-    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, tt.count.p, P, per, nsplit, nt.p);
+    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, tt.count.p, P, per, nsplit, meta, nt.p);
     BTK_LAUNCH_CHECK();
     exclusive_sum_i32(nt.p, tb.p, pct, s);
     int *h = reinterpret_cast<int *>(host_scratch().v);
@@ -215,7 +234,7 @@ void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, 
     tk.first.resize(m);
     tk.e0.resize(m);
     tk.e1.resize(m);
-    k_task_fill<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit,
+    k_task_fill<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit, meta,
                                          tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
     BTK_LAUNCH_CHECK();
 }

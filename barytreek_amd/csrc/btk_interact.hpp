@@ -308,11 +308,40 @@ struct Tasks {
     int n = 0;
     DBuf<int> panel, first, e0, e1;
 };
+// With meta, entries whose source count meta[e].z is zero are left out of the
+// balance (another kernel evaluates them).
 void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
-                 hipStream_t s);
+                 hipStream_t s, const int4 *meta = nullptr);
 // Entry slices per panel so that the launch has about target_items work
 // items (at most 16 slices per panel).
 int choose_split(long long base_items, long long target_items);
+
+// Mutual PP pairs of one tree (btk_mutual.hip): workgroups of `waves`
+// wavefronts, each item a block of targets of a leaf T and a slice of T's
+// owned mutual entries (mmeta: source descriptors, zero count if not owned).
+struct Mutual {
+    int n = 0;
+    int tpt = 1, waves = 1;
+    DBuf<int4> mmeta;
+    Tasks tasks;
+};
+// Finds the mutual pairs of a one-tree list, sets the source count of both
+// their entries in meta to zero and builds the work items.
+void build_mutual(const List &l, const Tree &t, int tpt, int4 *meta, Mutual &mu, hipStream_t s);
+// Mixed-precision evaluation of the mutual pairs, both directions, into out[C][n].
+template <int DIM, int KERNEL>
+void launch_mutual(const Mutual &mu, const Tree &t, const double *q_sorted, const KParams &kp, double *out,
+                   hipStream_t s);
+extern template void launch_mutual<2, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &,
+                                                      double *, hipStream_t);
+extern template void launch_mutual<2, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
+                                                           const KParams &, double *, hipStream_t);
+extern template void launch_mutual<3, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &,
+                                                      double *, hipStream_t);
+extern template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, const Tree &, const double *,
+                                                            const KParams &, double *, hipStream_t);
+extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
+                                                           const KParams &, double *, hipStream_t);
 
 // Launch of one interaction group with everything fixed but the kernel.
 struct InteractLaunch {

@@ -73,12 +73,13 @@ struct DBuf {
         if (count) p = static_cast<T *>(dmalloc(count * sizeof(T)));
         n = count;
     }
-    // keeps contents (first min(n, count) elements)
+    // keeps contents (first min(n, count) elements).  The old block goes back
+    // to the cache with the copy still queued on s; the cache gives it only to
+    // this thread's later work (queued behind the copy) until the call ends.
     void grow(size_t count, hipStream_t s) {
         if (count <= n) return;
         T *q = static_cast<T *>(dmalloc(count * sizeof(T)));
         if (p && n) BTK_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        BTK_HIP(hipStreamSynchronize(s));
         reset();
         p = q;
         n = count;

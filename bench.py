@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline", action="store_true",
+                    help="time the CPU oracle also on workloads above 1M points (minutes of CPU time)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
@@ -409,7 +411,7 @@ def main():
         ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
         res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
-                                   "far_tpt": t.far_tpt, "autotuned": not args.no_autotune}
+                                   "far_tpt": t.far_tpt, "mutual": t.mutual, "autotuned": not args.no_autotune}
         del out
     head = "mixed" if "mixed" in res else variants[0]
     main_r = res[head]
@@ -462,7 +464,9 @@ def main():
                                 traffic_source=os.path.relpath(tfile, HERE))
     if "plain" in res and "mixed" in res:
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
-    if not args.no_cpu_baseline:
+    # above 1M points the oracle's tree, traversal and far field alone take
+    # minutes, so the CPU leg runs there only when asked for
+    if not args.no_cpu_baseline and (n <= 1_000_000 or args.cpu_baseline):
         log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, args.cpu_stride)
         line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]

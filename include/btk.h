@@ -117,25 +117,27 @@ int btk_release_cached(void);
 
 /* Launch shapes of the interaction kernels (no reference counterpart; the
  * reference has one thread per pair).  Process-wide; zero fields mean the
- * built-in defaults.  Results do not depend on the shapes except through the
- * order of FP64 atomic adds when a panel's entries are split over several
- * work items. */
+ * built-in defaults.  In FP64, results depend on the shapes only through the
+ * order of FP64 atomic adds; in mixed precision the grouping of FP32 partial
+ * sums changes as well (differences at FP32 rounding level). */
 typedef struct btk_tuning {
     int near_items; /* work items the PP+PC launch aims for (entry slices are added to reach it) */
     int far_items;  /* the same for the CP+CC launch */
     int near_tpt;   /* targets per lane of the mixed-precision PP+PC kernel: 1 or 2 */
     int far_tpt;    /* the same for CP+CC */
-    int reserved[4];
+    int mutual;     /* 1 = mixed precision, one tree: a PP leaf pair listed in both orders, (T,S) and (S,T),
+                       is evaluated once for both directions; 0 = once per entry (default) */
+    int reserved[3];
 } btk_tuning;
 int btk_set_tuning(const btk_tuning *t);
 int btk_get_tuning(btk_tuning *t);
 
 /* Autotuner (no reference counterpart): builds the trees and list of one
  * problem (the arguments of btk_solve), times the PP+PC and CP+CC launches
- * under each candidate shape (work-item targets 8192..131072, and 1 or 2
- * targets per lane in mixed precision), installs the fastest with
- * btk_set_tuning and returns it in *chosen (may be null).  Writes no
- * caller memory besides *chosen. */
+ * under each candidate shape (work-item targets 8192..131072; in mixed
+ * precision also 1 or 2 targets per lane and mutual on or off), installs
+ * the fastest with btk_set_tuning and returns it in *chosen (may be null).
+ * Writes no caller memory besides *chosen. */
 int btk_autotune(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
                  const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
                  const double *d_q, int coincident, btk_tuning *chosen, void *stream);

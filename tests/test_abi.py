@@ -51,7 +51,7 @@ def test_tuning_round_trip_and_rejects_bad_values():
         with pytest.raises(btk.BtkError, match="work item"):
             btk.set_tuning(far_items=-1)
     finally:
-        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt)
+        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual)
 
 
 def test_components():

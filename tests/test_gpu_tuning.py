@@ -16,7 +16,7 @@ ITEMS = {8192, 16384, 32768, 65536, 131072}
 def _restore_tuning():
     old = btk.get_tuning()
     yield
-    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt)
+    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual)
 
 
 def _problem(dim, n, seed):
@@ -35,12 +35,13 @@ def test_autotune_installs_a_candidate(dim, prec):
     o = btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=prec)
     t = btk.autotune(cfg, o, d[0], d[1], d[2], d[3])
     g = btk.get_tuning()
-    assert (g.near_items, g.far_items, g.near_tpt, g.far_tpt) == (t.near_items, t.far_items, t.near_tpt, t.far_tpt)
+    assert (g.near_items, g.far_items, g.near_tpt, g.far_tpt, g.mutual) == \
+        (t.near_items, t.far_items, t.near_tpt, t.far_tpt, t.mutual)
     assert t.near_items in ITEMS and t.far_items in ITEMS
     if prec == btk.PREC_MIXED:
-        assert t.near_tpt in (1, 2) and t.far_tpt in (1, 2)
+        assert t.near_tpt in (1, 2) and t.far_tpt in (1, 2) and t.mutual in (0, 1)
     else:
-        assert t.near_tpt == 0 and t.far_tpt == 0
+        assert t.near_tpt == 0 and t.far_tpt == 0 and t.mutual == 0
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -51,18 +52,44 @@ def test_results_independent_of_launch_shapes(dim):
     C = btk.components(dim, btk.KERNEL_POT_GRAD)
     outs = {}
     for prec in (btk.PREC_FP64, btk.PREC_MIXED):
-        for shape in [(0, 0, 0, 0), (8192, 8192, 1, 1), (131072, 131072, 2, 2), (16384, 65536, 1, 2)]:
+        for shape in [(0, 0, 0, 0, 0), (8192, 8192, 1, 1, 0), (131072, 131072, 2, 2, 1), (16384, 65536, 1, 2, 1)]:
             btk.set_tuning(*shape)
             out = torch.zeros(C * 40000, dtype=torch.float64, device="cuda")
             btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=prec), d[0], d[1],
                       d[2], d[3], out)
             outs[(prec, shape)] = out
-    base = outs[(btk.PREC_FP64, (0, 0, 0, 0))]
+    base = outs[(btk.PREC_FP64, (0, 0, 0, 0, 0))]
     for (prec, shape), out in outs.items():
         rel = float(torch.linalg.norm(out - base) / torch.linalg.norm(base))
         # FP64: only the order of atomic adds differs; mixed: within its own tolerance of FP64
         assert rel < (1e-13 if prec == btk.PREC_FP64 else 1e-6), (prec, shape, rel)
-    mixed = [outs[k] for k in outs if k[0] == btk.PREC_MIXED]
-    for out in mixed[1:]:
-        # the mixed path rounds identically under every shape; only FP64 adds are reordered
-        assert float(torch.linalg.norm(out - mixed[0]) / torch.linalg.norm(mixed[0])) < 1e-13
+    fp64 = [outs[k] for k in outs if k[0] == btk.PREC_FP64]
+    for out in fp64[1:]:
+        assert float(torch.linalg.norm(out - fp64[0]) / torch.linalg.norm(fp64[0])) < 1e-13
+
+
+@pytest.mark.parametrize("dim,kernel", [(2, btk.KERNEL_POT), (2, btk.KERNEL_POT_GRAD), (3, btk.KERNEL_POT),
+                                        (3, btk.KERNEL_REF_VEL3D), (3, btk.KERNEL_POT_GRAD)])
+def test_mutual_pairs_match_one_way_evaluation(dim, kernel):
+    """Mixed precision: evaluating mutual PP pairs once for both directions
+    gives the one-way (per entry) result to FP32 rounding, for every kernel
+    and both lane shapes; both are within 1e-6 of FP64."""
+    print(f"\n[tuning] mutual dim={dim} kernel={kernel}", flush=True)
+    n = 40000
+    d = _problem(dim, n, 5)
+    cfg = btk.make_config(dim, 5, 250, 0.7, 0.0)
+    C = btk.components(dim, kernel)
+
+    def run(prec, **tu):
+        btk.set_tuning(**tu)
+        out = torch.zeros(C * n, dtype=torch.float64, device="cuda")
+        btk.solve(cfg, btk.options(kernel=kernel, exclude_self=1, precision=prec), d[0], d[1], d[2], d[3], out)
+        return out
+
+    ref = run(btk.PREC_FP64)
+    one_way = run(btk.PREC_MIXED)
+    for tpt in (1, 2):
+        mutual = run(btk.PREC_MIXED, near_tpt=tpt, mutual=1)
+        rel = float(torch.linalg.norm(mutual - one_way) / torch.linalg.norm(one_way))
+        assert rel < 1e-6, (tpt, rel)
+        assert float(torch.linalg.norm(mutual - ref) / torch.linalg.norm(ref)) < 1e-6
