@@ -44,53 +44,92 @@ __global__ void k_entry_meta(int n, const int *__restrict__ gsrc, const unsigned
 }
 
 // Entry slices balanced by source count (a leaf's points, or P proxies).
-__global__ void k_task_fill(int pct, const int *__restrict__ goff, int far, const int *__restrict__ gsrc,
-                            const unsigned char *__restrict__ gtype, const int *__restrict__ scount, int P, int per,
-                            int nsplit, const int4 *__restrict__ meta, const int *__restrict__ tcount,
-                            const int *__restrict__ ntask, const int *__restrict__ tbase, int *__restrict__ tpanel,
-                            int *__restrict__ tfirst, int *__restrict__ te0, int *__restrict__ te1) {
-    const int p = blockIdx.x * TB + threadIdx.x;
-    if (p >= pct || ntask[p] == 0) return;
-    const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
-    auto nsrc = [&](int e) -> long long { return meta ? meta[e].z : ((gtype[e] & 1) ? P : scount[gsrc[e]]); };
-    int ne = g1 - g0;
-    if (meta) {
-        int nz = 0;
-        for (int e = g0; e < g1; e++) nz += meta[e].z > 0;
-        ne = nz;
+// One wavefront per panel: the entries' source counts and their running sum
+// go to LDS, lane 0 places the slice boundaries, the lanes write the items.
+// Slice k closes after the first entry at which the running count reaches
+// (k + 1) / ns of the total, provided each later slice keeps at least one
+// entry; when no entry qualifies, slice k runs to the end and the later
+// slices are empty.
+constexpr int kTaskEntries = 4096;  // entries of one panel held in LDS
+
+__device__ __forceinline__ long long wave_incl_scan(long long v) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int d = 1; d < kLanes; d <<= 1) {
+        const long long u = __shfl_up(v, d, kLanes);
+        if (lane >= d) v += u;
     }
-    const int ns = min(nsplit, ne);
+    return v;
+}
+
+__global__ void __launch_bounds__(kLanes)
+    k_task_fill(int pct, const int *__restrict__ goff, int far, const int *__restrict__ gsrc,
+                const unsigned char *__restrict__ gtype, const int *__restrict__ scount, int P, int per, int nsplit,
+                const int4 *__restrict__ meta, const int *__restrict__ tcount, const int *__restrict__ ntask,
+                const int *__restrict__ tbase, int *__restrict__ tpanel, int *__restrict__ tfirst,
+                int *__restrict__ te0, int *__restrict__ te1) {
+    __shared__ long long cum[kTaskEntries];
+    __shared__ int bound[kMaxSplit + 1];
+    __shared__ int s_ns;
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (ntask[p] == 0) return;
+    const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
+    const int ne_all = g1 - g0;
+    auto nsrc = [&](int e) -> long long { return meta ? meta[e].z : ((gtype[e] & 1) ? P : scount[gsrc[e]]); };
+    long long carry = 0;
+    int nz = 0;
+    for (int c0 = 0; c0 < ne_all; c0 += kLanes) {
+        const int i = c0 + lane;
+        const long long v = i < ne_all ? nsrc(g0 + i) : 0;
+        const long long sc = wave_incl_scan(v);
+        if (i < ne_all && i < kTaskEntries) cum[i] = carry + sc;
+        carry += __shfl(sc, kLanes - 1, kLanes);
+        nz += __popcll(__ballot(i < ne_all && (!meta || v > 0)));
+    }
+    const long long total = carry;
+    if (lane == 0) {
+        const int ns = min(nsplit, nz);
+        bound[0] = g0;
+        int k = 0, prev = -1;  // prev: index of the entry that closed the last slice
+        for (; k < ns - 1; k++) {
+            int i = -1;
+            if (ne_all <= kTaskEntries) {
+                int lo = prev + 1, hi = ne_all;  // first i >= lo with cum[i] * ns >= (k + 1) * total
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cum[mid] * ns >= (k + 1) * total) hi = mid;
+                    else lo = mid + 1;
+                }
+                i = lo < ne_all ? lo : -1;
+            } else {  // more entries than LDS holds: running sum from memory
+                long long c = 0;
+                for (int j = 0; j < ne_all; j++) {
+                    c += nsrc(g0 + j);
+                    if (j > prev && c * ns >= (k + 1) * total) {
+                        i = j;
+                        break;
+                    }
+                }
+            }
+            if (i < 0 || (g1 - (g0 + i + 1)) < (ns - 1 - k)) break;
+            bound[k + 1] = g0 + i + 1;
+            prev = i;
+        }
+        for (k = k + 1; k <= ns; k++) bound[k] = g1;  // the open slice runs to g1, later ones are empty
+        s_ns = ns;
+    }
+    __syncthreads();
+    const int ns = s_ns;
     const int nt = far ? P : tcount[p];
     const int nb = (nt + per - 1) / per;
-    long long total = 0;
-    for (int e = g0; e < g1; e++) total += nsrc(e);
     const int base = tbase[p];
-    int k = 0, start = g0;
-    long long cum = 0;
-    for (int e = g0; e < g1 && k < ns - 1; e++) {
-        cum += nsrc(e);
-        // close slice k once it holds its share, leaving one entry for each later slice
-        if (cum * ns >= (k + 1) * total && (g1 - (e + 1)) >= (ns - 1 - k)) {
-            for (int b = 0; b < nb; b++) {
-                const int t = base + b * ns + k;
-                tpanel[t] = p;
-                tfirst[t] = b * per;
-                te0[t] = start;
-                te1[t] = e + 1;
-            }
-            start = e + 1;
-            k++;
-        }
-    }
-    for (; k < ns; k++) {  // the last slice (and any that found no boundary) ends at g1
-        for (int b = 0; b < nb; b++) {
-            const int t = base + b * ns + k;
-            tpanel[t] = p;
-            tfirst[t] = b * per;
-            te0[t] = start;
-            te1[t] = g1;
-        }
-        start = g1;
+    for (int i = lane; i < nb * ns; i += kLanes) {
+        const int bb = i / ns, k = i % ns;
+        tpanel[base + i] = p;
+        tfirst[base + i] = bb * per;
+        te0[base + i] = bound[k];
+        te1[base + i] = bound[k + 1];
     }
 }
 
@@ -234,8 +273,8 @@ void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, 
     tk.first.resize(m);
     tk.e0.resize(m);
     tk.e1.resize(m);
-    k_task_fill<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit, meta,
-                                         tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
+    k_task_fill<<<pct, kLanes, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit, meta,
+                                       tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
     BTK_LAUNCH_CHECK();
 }
 

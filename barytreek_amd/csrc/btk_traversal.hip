@@ -9,6 +9,8 @@
 // generation, and the output list is the reference's list in the reference's
 // order.  The MAC geometry (box centres, distance, ratio) is evaluated with
 // the reference's expressions in FP64 without contraction (-ffp-contract=off).
+#include <algorithm>
+
 #include "btk_device.hpp"
 
 namespace btk {
@@ -30,12 +32,8 @@ TreeView view(const Tree &t) {
 
 // code: 0..3 emit with that type, 4 refine target, 5 refine source
 template <int DIM>
-__global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__ cur, TreeView T, TreeView S,
-                                                 double theta, int thr, unsigned char *__restrict__ code,
-                                                 long long *__restrict__ val) {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= G) return;
-    const int it = cur[i].x, is = cur[i].y;
+__device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, const TreeView &S, double theta,
+                                             int thr) {
     // interaction_list.cpp:141-154
     const double x1 = 0.5 * (T.lo0[it] + T.hi0[it]);
     const double y1 = 0.5 * (T.lo1[it] + T.hi1[it]);
@@ -54,35 +52,33 @@ __global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__
     double sep = 100;
     if (dist > 0) sep = (T.radius[it] + S.radius[is]) / dist;
     const int ct = T.count[it], cs = S.count[is];
-    int c;
-    if (sep < theta) {  // :155-179
-        c = (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);
-    } else {  // :180-203
-        const bool lt = T.child0[it] < 0, lsf = S.child0[is] < 0;
-        if (lt && lsf)
-            c = 0;
-        else if (lt)
-            c = 5;
-        else if (lsf)
-            c = 4;
-        else
-            c = ct > cs ? 4 : 5;
-    }
+    if (sep < theta) return (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);  // :155-179
+    // :180-203
+    const bool lt = T.child0[it] < 0, lsf = S.child0[is] < 0;
+    if (lt && lsf) return 0;
+    if (lt) return 5;
+    if (lsf) return 4;
+    return ct > cs ? 4 : 5;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__ cur, TreeView T, TreeView S,
+                                                 double theta, int thr, unsigned char *__restrict__ code,
+                                                 long long *__restrict__ val) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= G) return;
+    const int c = classify_pair<DIM>(cur[i].x, cur[i].y, T, S, theta, thr);
     code[i] = (unsigned char)c;
     val[i] = c < 4 ? 1ll : (1ll << 32);
 }
 
+// Emitted pair or refined children of pair i, given ex = emitted count |
+// refined count << 32 of the pairs before it.
 template <int DIM>
-__global__ void __launch_bounds__(TB)
-    k_write(int G, const int2 *__restrict__ cur, const unsigned char *__restrict__ code,
-            const long long *__restrict__ val, const long long *__restrict__ inc, TreeView T, TreeView S, int lbase,
-            btk_interact_pair *__restrict__ out, int2 *__restrict__ next) {
+__device__ __forceinline__ void write_pair(int c, int it, int is, long long ex, const TreeView &T,
+                                           const TreeView &S, int lbase, btk_interact_pair *__restrict__ out,
+                                           int2 *__restrict__ next) {
     constexpr int NC = DIM == 3 ? 8 : 4;
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= G) return;
-    const long long ex = inc[i] - val[i];
-    const int c = code[i];
-    const int it = cur[i].x, is = cur[i].y;
     if (c < 4) {
         btk_interact_pair p;
         p.target_panel = it;
@@ -100,6 +96,67 @@ __global__ void __launch_bounds__(TB)
             for (int j = 0; j < NC; j++) next[NC * k + j] = make_int2(it, b + j);
         }
     }
+}
+
+// A small generation in one workgroup: classify, exclusive scan of the
+// packed counts, writes; the totals (emitted | refined << 32) go to *tot in
+// pinned host memory.
+constexpr int FB = 1024;
+constexpr int kFuseMax = 8192;
+
+template <int DIM>
+__global__ void __launch_bounds__(FB)
+    k_generation(int G, const int2 *__restrict__ cur, TreeView T, TreeView S, double theta, int thr, int lbase,
+                 btk_interact_pair *__restrict__ out, int2 *__restrict__ next, long long *tot) {
+    __shared__ long long wsum[FB / 64];
+    __shared__ long long btot;
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    long long carry = 0;
+    for (int c0 = 0; c0 < G; c0 += FB) {
+        const int i = c0 + tid;
+        int c = 0, it = 0, is = 0;
+        long long v = 0;
+        if (i < G) {
+            it = cur[i].x;
+            is = cur[i].y;
+            c = classify_pair<DIM>(it, is, T, S, theta, thr);
+            v = c < 4 ? 1ll : (1ll << 32);
+        }
+        long long incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long u = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += u;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            long long x = lane < FB / 64 ? wsum[lane] : 0;
+#pragma unroll
+            for (int d = 1; d < FB / 64; d <<= 1) {
+                const long long u = __shfl_up(x, d, 64);
+                if (lane >= d) x += u;
+            }
+            if (lane < FB / 64) wsum[lane] = x;  // inclusive over wavefronts
+            if (lane == FB / 64 - 1) btot = x;
+        }
+        __syncthreads();
+        const long long ex = carry + (w > 0 ? wsum[w - 1] : 0) + incl - v;
+        if (i < G) write_pair<DIM>(c, it, is, ex, T, S, lbase, out, next);
+        carry += btot;
+        __syncthreads();  // wsum and btot are rewritten by the next chunk
+    }
+    if (tid == 0) *tot = carry;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_write(int G, const int2 *__restrict__ cur, const unsigned char *__restrict__ code,
+            const long long *__restrict__ val, const long long *__restrict__ inc, TreeView T, TreeView S, int lbase,
+            btk_interact_pair *__restrict__ out, int2 *__restrict__ next) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= G) return;
+    write_pair<DIM>(code[i], cur[i].x, cur[i].y, inc[i] - val[i], T, S, lbase, out, next);
 }
 
 // Keys for the grouped forms: (far << tbits) | target.
@@ -163,39 +220,60 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
     BTK_HIP(hipMemcpyAsync(cur.p, roots.data(), sizeof(int2) * tt.nprob, hipMemcpyHostToDevice, s));
     int G = tt.nprob;
     int L = 0;
-    size_t lcap = 1024;
+    size_t lcap = std::max<size_t>(1024, 64 * (size_t)std::max(tt.pc, ts.pc));
     l->pairs.resize(lcap);
+    auto reserve = [&](size_t need) {
+        if (need <= lcap) return;
+        size_t ncap = lcap;
+        while (ncap < need) ncap *= 2;
+        l->pairs.grow(ncap, s);
+        lcap = ncap;
+    };
     while (G > 0) {
-        DBuf<unsigned char> code(G);
-        DBuf<long long> val(G), inc(G);
-        if (dim == 3)
-            k_classify<3><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, code.p,
-                                                 val.p);
-        else
-            k_classify<2><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, code.p,
-                                                 val.p);
-        BTK_LAUNCH_CHECK();
-        inclusive_sum_i64(val.p, inc.p, G, s);
-        BTK_HIP(hipMemcpyAsync(hv, inc.p + (G - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
-        BTK_HIP(hipStreamSynchronize(s));
-        const int E = (int)(hv[0] & 0xffffffffLL);
-        const long long X = hv[0] >> 32;
-        BTK_REQUIRE((long long)NC * X < (1ll << 31), "traversal generation exceeds 2^31 pairs");
-        if ((size_t)(L + E) > lcap) {
-            size_t ncap = lcap;
-            while (ncap < (size_t)(L + E)) ncap *= 2;
-            l->pairs.grow(ncap, s);
-            lcap = ncap;
+        long long tot;
+        if (G <= kFuseMax) {
+            // one launch; at most G pairs emitted and NC * G children
+            reserve((size_t)L + G);
+            nxt.resize((size_t)NC * G);
+            if (dim == 3)
+                k_generation<3><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, L,
+                                                 l->pairs.p, nxt.p, hv);
+            else
+                k_generation<2><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, L,
+                                                 l->pairs.p, nxt.p, hv);
+            BTK_LAUNCH_CHECK();
+            BTK_HIP(hipStreamSynchronize(s));
+            tot = hv[0];
+        } else {
+            DBuf<unsigned char> code(G);
+            DBuf<long long> val(G), inc(G);
+            if (dim == 3)
+                k_classify<3><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh,
+                                                     code.p, val.p);
+            else
+                k_classify<2><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh,
+                                                     code.p, val.p);
+            BTK_LAUNCH_CHECK();
+            inclusive_sum_i64(val.p, inc.p, G, s);
+            BTK_HIP(hipMemcpyAsync(hv, inc.p + (G - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
+            BTK_HIP(hipStreamSynchronize(s));
+            tot = hv[0];
+            const int E = (int)(tot & 0xffffffffLL);
+            const long long X = tot >> 32;
+            BTK_REQUIRE((long long)NC * X < (1ll << 31), "traversal generation exceeds 2^31 pairs");
+            reserve((size_t)L + E);
+            nxt.resize(X ? (size_t)NC * X : 1);
+            if (dim == 3)
+                k_write<3><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
+            else
+                k_write<2><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
+            BTK_LAUNCH_CHECK();
         }
-        const int Gn = (int)(NC * X);
-        nxt.resize(Gn ? Gn : 1);
-        if (dim == 3)
-            k_write<3><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
-        else
-            k_write<2><<<nblk(G), TB, 0, s>>>(G, cur.p, code.p, val.p, inc.p, T, S, L, l->pairs.p, nxt.p);
-        BTK_LAUNCH_CHECK();
+        const int E = (int)(tot & 0xffffffffLL);
+        const long long X = tot >> 32;
+        BTK_REQUIRE((long long)NC * X < (1ll << 31), "traversal generation exceeds 2^31 pairs");
         L += E;
-        G = Gn;
+        G = (int)(NC * X);
         std::swap(cur, nxt);
     }
     l->count = L;

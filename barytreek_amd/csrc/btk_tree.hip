@@ -403,19 +403,51 @@ __global__ void k_iota(int n, int *__restrict__ perm, int *__restrict__ point_le
 
 inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
 
+// The panel arrays of a tree, for one-launch copies.
+struct PanelArrays {
+    double *d[7];  // lo[3], hi[3], radius slot unused here
+    int *i[5];     // count, start, parent, child0, level
+};
+
+__global__ void k_copy_panels(int n, PanelArrays src, PanelArrays dst) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p >= n) return;
+#pragma unroll
+    for (int a = 0; a < 6; a++) dst.d[a][p] = src.d[a][p];
+#pragma unroll
+    for (int a = 0; a < 5; a++) dst.i[a][p] = src.i[a][p];
+}
+
 void grow_panels(Tree &t, int need, hipStream_t s) {
     if (need <= t.cap) return;
     int ncap = t.cap ? t.cap : 64;
     while (ncap < need) ncap *= 2;
+    DBuf<double> lo[3], hi[3];
+    DBuf<int> cnt(ncap), st(ncap), par(ncap), c0(ncap), lvl(ncap);
     for (int d = 0; d < 3; d++) {
-        t.lo[d].grow(ncap, s);
-        t.hi[d].grow(ncap, s);
+        lo[d].resize(ncap);
+        hi[d].resize(ncap);
     }
-    t.count.grow(ncap, s);
-    t.start.grow(ncap, s);
-    t.parent.grow(ncap, s);
-    t.child0.grow(ncap, s);
-    t.level.grow(ncap, s);
+    // the panels already in the old arrays, in one launch (none before the
+    // first allocation, when pc may already count the roots to be written)
+    const int live = std::min(t.pc, t.cap);
+    if (live > 0) {
+        PanelArrays a{{t.lo[0].p, t.lo[1].p, t.lo[2].p, t.hi[0].p, t.hi[1].p, t.hi[2].p, nullptr},
+                      {t.count.p, t.start.p, t.parent.p, t.child0.p, t.level.p}};
+        PanelArrays b{{lo[0].p, lo[1].p, lo[2].p, hi[0].p, hi[1].p, hi[2].p, nullptr},
+                      {cnt.p, st.p, par.p, c0.p, lvl.p}};
+        k_copy_panels<<<nblk(live), TB, 0, s>>>(live, a, b);
+        BTK_LAUNCH_CHECK();
+    }
+    for (int d = 0; d < 3; d++) {
+        t.lo[d] = std::move(lo[d]);
+        t.hi[d] = std::move(hi[d]);
+    }
+    t.count = std::move(cnt);
+    t.start = std::move(st);
+    t.parent = std::move(par);
+    t.child0 = std::move(c0);
+    t.level = std::move(lvl);
     t.cap = ncap;
 }
 
@@ -493,7 +525,10 @@ std::unique_ptr<Tree> build_forest(int dim, int thresh, const double *x, const d
     t->thresh = thresh;
     t->nprob = nprob;
     t->pc = nprob;
-    grow_panels(*t, std::max(64, nprob), s);
+    // room for a typical tree (a full tree with leaves of about thresh / 2^dim
+    // points) up front; deeper trees grow the arrays level by level
+    const long long guess = (2ll << dim) * (long long)n / std::max(1, thresh + 1) + 2ll * nprob + 64;
+    grow_panels(*t, (int)std::min<long long>(guess, 1 << 24), s);
     const size_t nn = n ? n : 1;
     t->perm.resize(nn);
     t->point_leaf.resize(nn);
