@@ -76,6 +76,8 @@ def parse():
                     help="time the CPU oracle also on workloads above 1M points (minutes of CPU time)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL",
+                    help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scale for the FETCH/WRITE_SIZE counters)")
@@ -403,7 +405,9 @@ def main():
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
         opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v])
         btk.set_tuning()  # defaults
-        if not args.no_autotune:  # untimed, like the warmup
+        if args.shapes:
+            btk.set_tuning(*[int(x) for x in args.shapes.split(",")])
+        elif not args.no_autotune:  # untimed, like the warmup
             log(f"autotune {v}")
             btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
         t = btk.get_tuning()
@@ -411,7 +415,8 @@ def main():
         ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
         res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
-                                   "far_tpt": t.far_tpt, "mutual": t.mutual, "autotuned": not args.no_autotune}
+                                   "far_tpt": t.far_tpt, "mutual": t.mutual,
+                                   "autotuned": not (args.no_autotune or args.shapes)}
         del out
     head = "mixed" if "mixed" in res else variants[0]
     main_r = res[head]
