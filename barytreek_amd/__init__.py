@@ -34,6 +34,7 @@ EXPORTED = [
     "btk_split_interactions", "btk_list_destroy", "btk_upward_pass_2d", "btk_upward_pass_3d",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
+    "btk_plan_create", "btk_plan_execute", "btk_plan_destroy",
 ]
 
 
@@ -143,6 +144,10 @@ def lib():
     L.btk_solve_multi.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, dp,
                                   C.POINTER(SolveStats), vp]
     L.btk_solve_batch.argtypes = [cfgp, optp, C.c_int, ip, dp, dp, dp, dp, dp, C.POINTER(SolveStats), vp]
+    L.btk_plan_create.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
+    L.btk_plan_execute.argtypes = [vp, dp, dp, vp]
+    L.btk_plan_destroy.argtypes = [vp]
+    L.btk_plan_destroy.restype = None
     _lib = L
     return L
 
@@ -411,3 +416,34 @@ def solve_batch(cfg, opts, offsets, x, y, z, q, out, stream=None) -> SolveStats:
                                  off.ctypes.data_as(C.POINTER(C.c_int)), _ptr(x), _ptr(y), _ptr(z), _ptr(q),
                                  _ptr(out), C.byref(st), _stream(stream)))
     return st
+
+
+class Plan:
+    """A solve plan for one geometry (btk_plan_create): execute(q, out) runs
+    the upward pass, interactions and downward pass for a new charge vector
+    as one HIP graph launch, adding into out[C][M]."""
+
+    def __init__(self, cfg, opts, xt, yt, zt, xs=None, ys=None, zs=None, stream=None):
+        coincident = xs is None
+        if coincident:
+            xs, ys, zs = xt, yt, zt
+        self._keep = (xt, yt, zt, xs, ys, zs)
+        self.h = C.c_void_p()
+        _check(lib().btk_plan_create(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
+                                     int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), int(coincident), _stream(stream),
+                                     C.byref(self.h)))
+
+    def execute(self, q, out, stream=None):
+        _check(lib().btk_plan_execute(self.h, _ptr(q), _ptr(out), _stream(stream)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().btk_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

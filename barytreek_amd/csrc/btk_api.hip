@@ -15,6 +15,31 @@ struct btk_tree {
 struct btk_list {
     std::unique_ptr<btk::List> l;
 };
+struct btk_plan {
+    btk_run_config cfg{};
+    btk_options o{};
+    int C = 1, P = 1, m = 0, n = 0;
+    std::unique_ptr<btk::Tree> ts, tt_own;
+    const btk::Tree *tt = nullptr;
+    std::unique_ptr<btk::List> l;
+    std::shared_ptr<btk::InteractPlan> ip;
+    btk::DBuf<double> q, W, Psi;
+    hipStream_t cs = nullptr;  // capture stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const double *g_q = nullptr;  // the pointers the graph was captured with
+    double *g_out = nullptr;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~btk_plan() {
+        drop_graph();
+        if (cs) (void)hipStreamDestroy(cs);
+    }
+};
 
 namespace {
 
@@ -298,7 +323,10 @@ int btk_tree_copy_panel_points_inside(const btk_tree *tree, int width, int *out)
     });
 }
 
-void btk_tree_destroy(btk_tree *tree) { delete tree; }
+void btk_tree_destroy(btk_tree *tree) {
+    delete tree;
+    btk::end_call();  // no work of the tree's is queued: every call synchronized
+}
 
 static int traversal(btk_run_config *cfg, int dim, const btk_tree *target, const btk_tree *source, void *stream,
                      btk_list **list) {
@@ -354,7 +382,10 @@ int btk_split_interactions(const btk_list *list, btk_interact_pair *pp, btk_inte
     });
 }
 
-void btk_list_destroy(btk_list *list) { delete list; }
+void btk_list_destroy(btk_list *list) {
+    delete list;
+    btk::end_call();
+}
 
 static int upward(const btk_run_config *cfg, int dim, const btk_tree *source, const double *d_charges, double *d_W,
                   void *stream) {
@@ -599,6 +630,71 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
         btk::tuning() = best;
         if (chosen) *chosen = best;
     });
+}
+
+int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
+                    const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                    const double *d_zs, int coincident, void *stream, btk_plan **plan) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in && plan, "null argument");
+        BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
+        auto p = std::make_unique<btk_plan>();
+        p->cfg = *cfg_in;
+        const int dim = p->cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        p->o = opts_or_default(opts);
+        hipStream_t s = as_stream(stream);
+        p->ts = btk::build_tree(dim, p->cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
+        if (!coincident) p->tt_own = btk::build_tree(dim, p->cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
+        p->tt = coincident ? p->ts.get() : p->tt_own.get();
+        p->l = btk::build_list(&p->cfg, *p->tt, *p->ts, s);
+        p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s);
+        p->m = m;
+        p->n = n;
+        p->P = btk::ipow(p->cfg.interp_degree + 1, dim);
+        p->C = btk::components(dim, p->o.kernel);
+        p->q.resize(n ? n : 1);
+        p->W.resize((size_t)p->ts->pc * p->P);
+        p->Psi.resize((size_t)p->C * p->tt->pc * p->P);
+        BTK_HIP(hipStreamCreateWithFlags(&p->cs, hipStreamNonBlocking));
+        BTK_HIP(hipStreamSynchronize(s));
+        *plan = p.release();
+    });
+}
+
+int btk_plan_execute(btk_plan *p, const double *d_q, double *d_out, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(p, "null plan");
+        hipStream_t s = as_stream(stream);
+        if (!p->exec || d_q != p->g_q || d_out != p->g_out) {
+            p->drop_graph();
+            // launches only: every buffer belongs to the plan or the caller
+            BTK_HIP(hipStreamBeginCapture(p->cs, hipStreamCaptureModeThreadLocal));
+            try {
+                btk::gather_sorted(*p->ts, d_q, p->q.p, p->cs);
+                btk::upward_pass(p->cfg, *p->ts, p->q.p, p->W.p, p->cs);
+                btk::zero_fill(p->Psi.p, p->Psi.n, p->cs);
+                btk::launch_interactions(*p->ip, p->q.p, p->W.p, d_out, p->Psi.p, p->cs);
+                btk::downward_pass(p->cfg, p->o, *p->tt, p->Psi.p, d_out, p->cs);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(p->cs, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            BTK_HIP(hipStreamEndCapture(p->cs, &p->graph));
+            BTK_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+            p->g_q = d_q;
+            p->g_out = d_out;
+        }
+        BTK_HIP(hipGraphLaunch(p->exec, s));
+        BTK_HIP(hipStreamSynchronize(s));
+    });
+}
+
+void btk_plan_destroy(btk_plan *plan) {
+    delete plan;
+    btk::end_call();  // btk_plan_execute synchronized, so nothing reads the plan's buffers
 }
 
 int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int nprob, const int *offsets,

@@ -134,73 +134,44 @@ __global__ void __launch_bounds__(kLanes)
 }
 
 template <int DIM, int KERNEL>
-void run_group(bool far, const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt,
-               const Tree &ts, const List &l, const SrcView &sv, const NodeTables &tabs, int4 *meta, double *dst,
-               const double *q_sorted, hipStream_t s, KernelEvents *kev) {
-    const int n1 = cfg.interp_degree + 1;
+void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
+                hipStream_t s, KernelEvents *kev) {
+    const Tree &tt = *p.tt, &ts = *p.ts;
+    const int n1 = p.cfg.interp_degree + 1;
     const int P = ipow(n1, DIM);
-    const bool f32 = o.precision == BTK_PREC_MIXED;
-    const btk_tuning &tu = tuning();
-    const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
-    // default: two targets per lane where the kernel is short (2D), one in 3D
-    const int tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((DIM == 3 || (far && P <= kLanes)) ? 1 : 2));
-    const int per = kLanes * tpt;
-    const int n_clusters = tt.pc - tt.n_leaves;
-    const long long base = far ? (long long)n_clusters * ((P + per - 1) / per)
-                               : (long long)tt.n / per + tt.n_leaves;
-    const long long items = far ? (tu.far_items ? tu.far_items : kFarItems) : (tu.near_items ? tu.near_items : kNearItems);
-    const int nsplit = choose_split(base, items);
-    // mixed precision, one tree, when asked for: mutual PP pairs once for both directions
-    Mutual mu;
-    const bool mutual = !far && f32 && &tt == &ts && tu.mutual;
-    if (mutual) build_mutual(l, tt, tpt, meta, mu, s);
-    Tasks tk;
-    build_tasks(l, tt, ts, far ? 1 : 0, P, per, nsplit, tk, s, mutual ? meta : nullptr);
-    InteractLaunch L;
-    L.tk = TaskView{tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p};
-    L.ev = EntryView{meta};
-    L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
-                   tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
-    L.sv = sv;
-    L.n_tasks = tk.n;
-    L.n1 = n1;
-    L.tpt = tpt;
-    L.far = far ? 1 : 0;
-    L.excl = o.exclude_self ? 1 : 0;
-    L.f32 = f32 ? 1 : 0;
-    L.atomic = nsplit > 1 ? 1 : 0;
-    // far targets are proxy points, which no zero-distance rule ties to a panel
-    L.same_tree = (&tt == &ts && !far) ? 1 : 0;
-    L.tabs = tabs;
-    L.kp = kp;
-    L.out = dst;
-    L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
-    if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
-    if (mutual) launch_mutual<DIM, KERNEL>(mu, tt, q_sorted, kp, dst, s);
-    launch_interact<DIM, KERNEL>(L, s);
-    if (kev) {
-        BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
-        (far ? kev->far_done : kev->near_done) = true;
+    const SrcView sv{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
+                     ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W};
+    for (int far = 0; far < 2; far++) {
+        const InteractPlan::Group &g = p.g[far];
+        if (!g.present) continue;
+        double *dst = far ? Psi : out;
+        InteractLaunch L;
+        L.tk = TaskView{g.tk.panel.p, g.tk.first.p, g.tk.e0.p, g.tk.e1.p};
+        L.ev = EntryView{p.meta.p};
+        L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
+                       tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
+        L.sv = sv;
+        L.n_tasks = g.tk.n;
+        L.n1 = n1;
+        L.tpt = g.tpt;
+        L.far = far;
+        L.excl = p.o.exclude_self ? 1 : 0;
+        L.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
+        L.atomic = g.nsplit > 1 ? 1 : 0;
+        // far targets are proxy points, which no zero-distance rule ties to a panel
+        L.same_tree = (&tt == &ts && !far) ? 1 : 0;
+        L.tabs = p.tabs;
+        L.kp = p.kp;
+        L.out = dst;
+        L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
+        if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
+        if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, s);
+        launch_interact<DIM, KERNEL>(L, s);
+        if (kev) {
+            BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
+            (far ? kev->far_done : kev->near_done) = true;
+        }
     }
-}
-
-template <int DIM, int KERNEL>
-void run_all(const btk_run_config &cfg, const btk_options &o, const KParams &kp, const Tree &tt, const Tree &ts,
-             const List &l, const SrcView &sv, const NodeTables &tabs, double *out, double *Psi, hipStream_t s,
-             KernelEvents *kev) {
-    DBuf<int4> meta(l.count ? l.count : 1);
-    if (l.count) {
-        k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p,
-                                                  ipow(cfg.interp_degree + 1, DIM), meta.p);
-        BTK_LAUNCH_CHECK();
-    }
-    if (l.counts[0] + l.counts[1] > 0)
-        run_group<DIM, KERNEL>(false, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, out, sv.q, s, kev);
-    if (l.counts[2] + l.counts[3] > 0)
-        run_group<DIM, KERNEL>(true, cfg, o, kp, tt, ts, l, sv, tabs, meta.p, Psi, sv.q, s, kev);
-    // meta goes back to the cache on return, while the launches may still run;
-    // the cache hands it out only to this thread's later work on this stream
-    // until the API call has synchronized
 }
 
 // ---------------------------------------------------------------------------
@@ -289,31 +260,78 @@ int choose_split(long long base_items, long long target_items) {
     return (int)std::max(1ll, std::min<long long>(kMaxSplit, k));
 }
 
-void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts, const List &l,
-                      const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s,
-                      KernelEvents *kev) {
+std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt,
+                                                const Tree &ts, const List &l, hipStream_t s) {
     BTK_REQUIRE(tt.dim == ts.dim, "target and source trees differ in dimension");
+    const int dim = tt.dim;
     const int n1 = cfg.interp_degree + 1;
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
     BTK_REQUIRE(o.precision == BTK_PREC_FP64 || o.precision == BTK_PREC_MIXED, "unknown precision");
-    const KParams kp = make_kparams(tt.dim, cfg.ker_eps, o);
-    const NodeTables tabs = node_tables(cfg.interp_degree);
-    const SrcView sv{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
-                     ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W};
-    if (tt.dim == 3) {
-        if (o.kernel == BTK_KERNEL_POT)
-            run_all<3, BTK_KERNEL_POT>(cfg, o, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
-        else if (o.kernel == BTK_KERNEL_REF_VEL3D)
-            run_all<3, BTK_KERNEL_REF_VEL3D>(cfg, o, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
-        else
-            run_all<3, BTK_KERNEL_POT_GRAD>(cfg, o, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
-    } else {
-        BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "the reference vel kernel is 3D only");
-        if (o.kernel == BTK_KERNEL_POT)
-            run_all<2, BTK_KERNEL_POT>(cfg, o, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
-        else
-            run_all<2, BTK_KERNEL_POT_GRAD>(cfg, o, kp, tt, ts, l, sv, tabs, out, Psi, s, kev);
+    BTK_REQUIRE(!(dim == 2 && o.kernel == BTK_KERNEL_REF_VEL3D), "the reference vel kernel is 3D only");
+    auto p = std::make_shared<InteractPlan>();
+    p->cfg = cfg;
+    p->o = o;
+    p->tt = &tt;
+    p->ts = &ts;
+    p->kp = make_kparams(dim, cfg.ker_eps, o);
+    p->tabs = node_tables(cfg.interp_degree);
+    const int P = ipow(n1, dim);
+    const bool f32 = o.precision == BTK_PREC_MIXED;
+    const btk_tuning &tu = tuning();
+    p->meta.resize(l.count ? l.count : 1);
+    if (l.count) {
+        k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p, P,
+                                                  p->meta.p);
+        BTK_LAUNCH_CHECK();
     }
+    for (int far = 0; far < 2; far++) {
+        InteractPlan::Group &g = p->g[far];
+        g.present = l.counts[2 * far] + l.counts[2 * far + 1] > 0;
+        if (!g.present) continue;
+        const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
+        // default: two targets per lane where the kernel is short (2D), one in 3D
+        g.tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((dim == 3 || (far && P <= kLanes)) ? 1 : 2));
+        const int per = kLanes * g.tpt;
+        const int n_clusters = tt.pc - tt.n_leaves;
+        const long long base = far ? (long long)n_clusters * ((P + per - 1) / per)
+                                   : (long long)tt.n / per + tt.n_leaves;
+        const long long items = far ? (tu.far_items ? tu.far_items : kFarItems)
+                                    : (tu.near_items ? tu.near_items : kNearItems);
+        g.nsplit = choose_split(base, items);
+        // mixed precision, one tree, when asked for: mutual PP pairs once for both directions
+        g.mutual = !far && f32 && &tt == &ts && tu.mutual;
+        if (g.mutual) build_mutual(l, tt, g.tpt, p->meta.p, g.mu, s);
+        build_tasks(l, tt, ts, far, P, per, g.nsplit, g.tk, s, g.mutual ? p->meta.p : nullptr);
+    }
+    return p;
+}
+
+void launch_interactions(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
+                         hipStream_t s, KernelEvents *kev) {
+    const int dim = p.tt->dim, kernel = p.o.kernel;
+    if (dim == 3) {
+        if (kernel == BTK_KERNEL_POT)
+            launch_all<3, BTK_KERNEL_POT>(p, q_sorted, W, out, Psi, s, kev);
+        else if (kernel == BTK_KERNEL_REF_VEL3D)
+            launch_all<3, BTK_KERNEL_REF_VEL3D>(p, q_sorted, W, out, Psi, s, kev);
+        else
+            launch_all<3, BTK_KERNEL_POT_GRAD>(p, q_sorted, W, out, Psi, s, kev);
+    } else {
+        if (kernel == BTK_KERNEL_POT)
+            launch_all<2, BTK_KERNEL_POT>(p, q_sorted, W, out, Psi, s, kev);
+        else
+            launch_all<2, BTK_KERNEL_POT_GRAD>(p, q_sorted, W, out, Psi, s, kev);
+    }
+}
+
+void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts, const List &l,
+                      const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s,
+                      KernelEvents *kev) {
+    // the plan's buffers go back to the cache on return, while the launches
+    // may still run; the cache hands them out only to this thread's later
+    // work on this stream until the API call has synchronized
+    const auto p = plan_interactions(cfg, o, tt, ts, l, s);
+    launch_interactions(*p, q_sorted, W, out, Psi, s, kev);
 }
 
 void direct_sum(int dim, const btk_options &o, double eps, int m, const int *idx, const double *xt, const double *yt,

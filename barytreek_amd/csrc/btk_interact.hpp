@@ -343,6 +343,27 @@ extern template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, cons
 extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
                                                            const KParams &, double *, hipStream_t);
 
+// Everything the interaction launches of one list need besides the charges,
+// the weights and the outputs: entry descriptors and the work items of the
+// near (PP+PC) and far (CP+CC) groups.  Built once (allocations, host
+// reads); launch_interactions then only launches kernels, so it can be
+// captured in a HIP graph.
+struct InteractPlan {
+    btk_run_config cfg{};
+    btk_options o{};
+    const Tree *tt = nullptr, *ts = nullptr;
+    KParams kp{};
+    NodeTables tabs{};
+    DBuf<int4> meta;
+    struct Group {
+        bool present = false;
+        int tpt = 1, nsplit = 1;
+        bool mutual = false;
+        Tasks tk;
+        Mutual mu;
+    } g[2];
+};
+
 // Launch of one interaction group with everything fixed but the kernel.
 struct InteractLaunch {
     TaskView tk;

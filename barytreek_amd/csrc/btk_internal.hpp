@@ -87,6 +87,10 @@ struct DBuf {
     T *get() const { return p; }
 };
 
+// Zeroes n doubles with a kernel (unlike hipMemsetAsync, safe to capture in
+// a HIP graph that is replayed).
+void zero_fill(double *p, size_t n, hipStream_t s);
+
 // Small pinned host staging buffer for scalar read-backs.
 struct HostScratch {
     long long *v = nullptr;
@@ -191,6 +195,13 @@ struct KernelEvents {
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
                       const List &l, const double *q_sorted, const double *W, double *out, double *Psi,
                       hipStream_t s, KernelEvents *kev = nullptr);
+// The same in two steps: the plan (descriptors and work items; allocates and
+// reads counts back) and the launches alone.  The plan refers to tt, ts.
+struct InteractPlan;
+std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt,
+                                                const Tree &ts, const List &l, hipStream_t s);
+void launch_interactions(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
+                         hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
                    hipStream_t s);
 

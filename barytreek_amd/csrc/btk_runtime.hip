@@ -3,6 +3,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <algorithm>
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -113,6 +114,20 @@ void release_cached() {
     }
     g_free.clear();
     g_cached.clear();
+}
+
+namespace {
+__global__ void k_zero(size_t n, double *__restrict__ p) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+}  // namespace
+
+void zero_fill(double *p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    k_zero<<<(unsigned)blocks, 256, 0, s>>>(n, p);
+    BTK_HIP(hipGetLastError());
 }
 
 HostScratch::HostScratch() { BTK_HIP(hipHostMalloc(reinterpret_cast<void **>(&v), 64 * sizeof(long long))); }

@@ -43,6 +43,9 @@ WORKLOADS = {
     "3d-sphere-plummer-4M": (3, 4_000_000, "sphere3", 8, 0.7, 2000, 2,
                              "3D potential+gradient, N=M=4,000,000 (half unit sphere, half Plummer a=0.3 r<=2), "
                              "q~U[-1,1], degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
+    "3d-plummer-16M": (3, 16_000_000, "plummer3", 8, 0.7, 2000, 2,
+                       "3D potential+gradient, N=M=16,000,000 Plummer a=0.3 truncated at r<=10, q~U[-1,1], "
+                       "degree 8, theta 0.7 (the fixed MAC), leaf 2000, eps 0, self excluded"),
 }
 
 
@@ -376,6 +379,8 @@ def main():
         x, y, z, q = wl.gaussian_mixture_2d(n, SEED)
     elif gen == "cube3":
         x, y, z, q = wl.uniform_cube(n, 3, SEED)
+    elif gen == "plummer3":
+        x, y, z, q = wl.plummer_3d(n, SEED, a=0.3, rmax=10.0)
     else:
         x, y, z, q = wl.sphere_plummer_3d(n, SEED)
     d = [None if a is None else torch.from_numpy(a).to(dev) for a in (x, y, z, q)]

@@ -247,6 +247,21 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
                     const double *d_q, int nrhs, int coincident, double *d_out, btk_solve_stats *stats,
                     void *stream);
 
+/* A solve plan for one geometry (no reference counterpart; for iterative
+ * methods that apply one operator to many vectors): the trees, the list and
+ * the interaction work items are built once, with the tuning current at
+ * creation.  btk_plan_execute then runs gather, upward pass, interactions
+ * and downward pass for one charge vector d_q[n] as a single HIP graph launch
+ * on `stream`, adding into d_out[C][m]; the graph is captured (on a stream
+ * the plan owns) at the first call and again whenever d_q or d_out differs
+ * from the previous call.  Calls on one plan must not overlap. */
+typedef struct btk_plan btk_plan;
+int btk_plan_create(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
+                    const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
+                    int coincident, void *stream, btk_plan **plan);
+int btk_plan_execute(btk_plan *plan, const double *d_q, double *d_out, void *stream);
+void btk_plan_destroy(btk_plan *plan);
+
 /* Many independent problems in one solve (no reference counterpart): problem
  * p owns points [offsets[p], offsets[p+1]) of d_x/d_y/d_z/d_q (targets =
  * sources within a problem; offsets is a host array of nprob + 1 entries,
