@@ -35,6 +35,8 @@ EXPORTED = [
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
     "btk_plan_create", "btk_plan_execute", "btk_plan_destroy",
+    "btk_mac_sample", "btk_mac_train", "btk_mac_set_weights", "btk_mac_get_weights", "btk_mac_set_budget",
+    "btk_mac_forward", "btk_mac_gradients",
 ]
 
 
@@ -59,7 +61,7 @@ class TreeInfo(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
-                ("deepest_only", C.c_int), ("precision", C.c_int), ("reserved", C.c_int * 3)]
+                ("deepest_only", C.c_int), ("precision", C.c_int), ("mac", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class Tuning(C.Structure):
@@ -144,6 +146,15 @@ def lib():
     L.btk_solve_multi.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, dp,
                                   C.POINTER(SolveStats), vp]
     L.btk_solve_batch.argtypes = [cfgp, optp, C.c_int, ip, dp, dp, dp, dp, dp, C.POINTER(SolveStats), vp]
+    fp = C.c_void_p
+    L.btk_mac_sample.argtypes = [cfgp, C.c_int, dp, dp, dp, C.c_int, C.c_uint, fp, fp, vp, C.POINTER(C.c_int), vp]
+    L.btk_mac_train.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, C.c_int,
+                                C.POINTER(C.c_float), vp]
+    L.btk_mac_set_weights.argtypes = [C.POINTER(C.c_float), C.c_int]
+    L.btk_mac_get_weights.argtypes = [C.POINTER(C.c_float), C.c_int]
+    L.btk_mac_set_budget.argtypes = [C.c_double]
+    L.btk_mac_forward.argtypes = [fp, C.c_int, fp, vp]
+    L.btk_mac_gradients.argtypes = [fp, fp, C.c_int, fp, C.POINTER(C.c_float), vp]
     L.btk_plan_create.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
     L.btk_plan_execute.argtypes = [vp, dp, dp, vp]
     L.btk_plan_destroy.argtypes = [vp]
@@ -197,10 +208,10 @@ def make_config(dim, degree, thresh, theta=0.7, eps=-1.0) -> RunConfig:
     return c
 
 
-def options(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, precision=PREC_FP64) -> Options:
+def options(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, precision=PREC_FP64, mac=0) -> Options:
     o = Options()
     o.kernel, o.convention, o.exclude_self = kernel, convention, exclude_self
-    o.deepest_only, o.precision = deepest_only, precision
+    o.deepest_only, o.precision, o.mac = deepest_only, precision, mac
     return o
 
 
@@ -447,3 +458,52 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Learned MAC (btk_mac_*)
+# ---------------------------------------------------------------------------
+MAC_FEATURES, MAC_OUTPUTS, MAC_WEIGHTS = 10, 12, 5644
+
+
+def mac_sample(cfg, x, y, z, n_samples, seed, X, Y, pairs=None, stream=None) -> int:
+    """Training samples from the treecode's own traversal of (x, y, z):
+    X[n][10] features, Y[n][12] measured log10(errors)/4, optionally the
+    (target, source) panels pairs[n][2]; returns the count."""
+    cnt = C.c_int()
+    _check(lib().btk_mac_sample(C.byref(cfg), int(x.numel()), _ptr(x), _ptr(y), _ptr(z), int(n_samples),
+                                int(seed), _ptr(X), _ptr(Y), _ptr(pairs), C.byref(cnt), _stream(stream)))
+    return cnt.value
+
+
+def mac_train(X, Y, n, epochs=50, batch=1024, lr=2e-3, seed=1, reinit=False, stream=None) -> float:
+    loss = C.c_float()
+    _check(lib().btk_mac_train(_ptr(X), _ptr(Y), int(n), int(epochs), int(batch), float(lr), int(seed),
+                               int(bool(reinit)), C.byref(loss), _stream(stream)))
+    return loss.value
+
+
+def mac_set_weights(w: np.ndarray) -> None:
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    _check(lib().btk_mac_set_weights(w.ctypes.data_as(C.POINTER(C.c_float)), int(w.size)))
+
+
+def mac_get_weights() -> np.ndarray:
+    w = np.zeros(MAC_WEIGHTS, np.float32)
+    _check(lib().btk_mac_get_weights(w.ctypes.data_as(C.POINTER(C.c_float)), MAC_WEIGHTS))
+    return w
+
+
+def mac_set_budget(rel_err: float) -> None:
+    _check(lib().btk_mac_set_budget(float(rel_err)))
+
+
+def mac_forward(X, n, out, stream=None):
+    _check(lib().btk_mac_forward(_ptr(X), int(n), _ptr(out), _stream(stream)))
+    return out
+
+
+def mac_gradients(X, Y, n, grad, stream=None) -> float:
+    loss = C.c_float()
+    _check(lib().btk_mac_gradients(_ptr(X), _ptr(Y), int(n), _ptr(grad), C.byref(loss), _stream(stream)))
+    return loss.value

@@ -32,6 +32,7 @@ SOURCES = [
     "btk_ik_3d_grad.hip",
     "btk_mrhs.hip",
     "btk_mutual.hip",
+    "btk_mac.hip",
     "btk_api.hip",
 ]
 

@@ -501,7 +501,7 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
         BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
         BTK_HIP(hipStreamSynchronize(s));
         const auto t1 = clk::now();
-        auto l = btk::build_list(&cfg, tt, *ts, s);
+        auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
         const int P = btk::ipow(cfg.interp_degree + 1, dim);
@@ -560,7 +560,7 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
         std::unique_ptr<btk::Tree> tt_own;
         if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
         const btk::Tree &tt = coincident ? *ts : *tt_own;
-        auto l = btk::build_list(&cfg, tt, *ts, s);
+        auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
         const int P = btk::ipow(cfg.interp_degree + 1, dim);
         const int C = btk::components(dim, o.kernel);
         btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P), out((size_t)C * (m ? m : 1));
@@ -632,6 +632,57 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
     });
 }
 
+int btk_mac_sample(const btk_run_config *cfg, int n, const double *d_x, const double *d_y, const double *d_z,
+                   int n_samples, unsigned seed, float *d_X, float *d_Y, int *d_pairs, int *count, void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg && count, "null argument");
+        *count = btk::mac_sample(*cfg, n, d_x, d_y, d_z, n_samples, seed, d_X, d_Y, d_pairs, as_stream(stream));
+    });
+}
+
+int btk_mac_train(const float *d_X, const float *d_Y, int n, int epochs, int batch, float lr, unsigned seed,
+                  int reinit, float *loss, void *stream) {
+    return guarded([&] {
+        const float l = btk::mac_train(d_X, d_Y, n, epochs, batch, lr, seed, reinit != 0, as_stream(stream));
+        if (loss) *loss = l;
+    });
+}
+
+int btk_mac_set_weights(const float *h_weights, int count) {
+    return guarded([&] {
+        BTK_REQUIRE(h_weights && count == BTK_MAC_WEIGHTS, "expected BTK_MAC_WEIGHTS weights");
+        btk::mac_set_weights(h_weights, nullptr);
+    });
+}
+
+int btk_mac_get_weights(float *h_weights, int count) {
+    return guarded([&] {
+        BTK_REQUIRE(h_weights && count == BTK_MAC_WEIGHTS, "expected BTK_MAC_WEIGHTS weights");
+        btk::mac_get_weights(h_weights, nullptr);
+    });
+}
+
+int btk_mac_set_budget(double rel_err) {
+    return guarded([&] {
+        BTK_REQUIRE(rel_err > 0 && rel_err < 1, "the budget is a relative error in (0, 1)");
+        btk::mac_model().budget = rel_err;
+    });
+}
+
+int btk_mac_forward(const float *d_X, int n, float *d_out, void *stream) {
+    return guarded([&] {
+        btk::mac_forward(d_X, n, d_out, as_stream(stream));
+        BTK_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int btk_mac_gradients(const float *d_X, const float *d_Y, int n, float *d_grad, float *loss, void *stream) {
+    return guarded([&] {
+        const float l = btk::mac_gradients(d_X, d_Y, n, d_grad, as_stream(stream));
+        if (loss) *loss = l;
+    });
+}
+
 int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
                     const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
                     const double *d_zs, int coincident, void *stream, btk_plan **plan) {
@@ -647,7 +698,7 @@ int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m
         p->ts = btk::build_tree(dim, p->cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
         if (!coincident) p->tt_own = btk::build_tree(dim, p->cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
         p->tt = coincident ? p->ts.get() : p->tt_own.get();
-        p->l = btk::build_list(&p->cfg, *p->tt, *p->ts, s);
+        p->l = btk::build_list(&p->cfg, *p->tt, *p->ts, s, p->o.mac != 0);
         p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s);
         p->m = m;
         p->n = n;
@@ -720,7 +771,7 @@ int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int n
         auto t = btk::build_forest(dim, cfg.fmm_cluster_thresh, d_x, d_y, d_z, off, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t1 = clk::now();
-        auto l = btk::build_list(&cfg, *t, *t, s);
+        auto l = btk::build_list(&cfg, *t, *t, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
         const int P = btk::ipow(cfg.interp_degree + 1, dim);
@@ -786,7 +837,7 @@ int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m
         const btk::Tree &tt = coincident ? *ts : *tt_own;
         BTK_HIP(hipStreamSynchronize(s));
         const auto t1 = clk::now();
-        auto l = btk::build_list(&cfg, tt, *ts, s);
+        auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
         const int P = btk::ipow(cfg.interp_degree + 1, dim);

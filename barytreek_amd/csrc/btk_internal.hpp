@@ -36,6 +36,8 @@ struct Error : std::runtime_error {
 // ---------------------------------------------------------------------------
 // Device memory: a size-bucketed cache in front of hipMalloc, so that
 // repeated solves of the same shape allocate nothing after the first.
+// No DBuf may live in static storage: it would be freed at exit, after the
+// cache's thread-local bookkeeping has been destroyed.
 // ---------------------------------------------------------------------------
 void *dmalloc(size_t bytes);
 void dfree(void *p);
@@ -183,7 +185,10 @@ std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const dou
 // panel ids interleaved level by level across the problems.
 std::unique_ptr<Tree> build_forest(int dim, int thresh, const double *x, const double *y, const double *z,
                                    const std::vector<int> &offsets, hipStream_t s);
-std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &t, const Tree &s, hipStream_t st);
+// learned_mac: consult the installed learned MAC where the theta MAC would
+// refine (3D).  visited: when given, receives every pair the traversal visits.
+std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &t, const Tree &s, hipStream_t st,
+                                 bool learned_mac = false, std::vector<int2> *visited = nullptr);
 void upward_pass(const btk_run_config &cfg, const Tree &src, const double *q_sorted, double *W, hipStream_t s);
 void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_t s);
 // Optional HIP events recorded around the near (PP+PC) and far (CP+CC)
@@ -218,6 +223,22 @@ void fmm_interactions_multi(const btk_run_config &cfg, const btk_options &o, con
 void direct_sum(int dim, const btk_options &o, double eps, int m, const int *idx, const double *xt,
                 const double *yt, const double *zt, int n, const double *xs, const double *ys,
                 const double *zs, const double *q, double *out, hipStream_t s);
+
+// Learned MAC (btk_mac.hip): the process-wide model and its training.
+struct MacModel {
+    DBuf<float> w;  // BTK_MAC_WEIGHTS parameters
+    bool ready = false;
+    double budget = 1e-7;
+};
+MacModel &mac_model();
+void mac_set_weights(const float *h, hipStream_t s);
+void mac_get_weights(float *h, hipStream_t s);
+int mac_sample(const btk_run_config &cfg, int n, const double *x, const double *y, const double *z, int n_samples,
+               unsigned seed, float *X, float *Y, int *pairs, hipStream_t s);
+float mac_train(const float *X, const float *Y, int n, int epochs, int batch, float lr, unsigned seed, bool reinit,
+                hipStream_t s);
+void mac_forward(const float *X, int n, float *out, hipStream_t s);
+float mac_gradients(const float *X, const float *Y, int n, float *grad, hipStream_t s);
 
 inline int ipow(int b, int e) {
     int r = 1;

@@ -11,7 +11,7 @@
 // the reference's expressions in FP64 without contraction (-ffp-contract=off).
 #include <algorithm>
 
-#include "btk_device.hpp"
+#include "btk_mac.hpp"
 
 namespace btk {
 namespace {
@@ -31,9 +31,9 @@ TreeView view(const Tree &t) {
 }
 
 // code: 0..3 emit with that type, 4 refine target, 5 refine source
-template <int DIM>
+template <int DIM, bool MAC>
 __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, const TreeView &S, double theta,
-                                             int thr) {
+                                             int thr, const MacView &mac, const float *mw) {
     // interaction_list.cpp:141-154
     const double x1 = 0.5 * (T.lo0[it] + T.hi0[it]);
     const double y1 = 0.5 * (T.lo1[it] + T.hi1[it]);
@@ -53,6 +53,11 @@ __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, 
     if (dist > 0) sep = (T.radius[it] + S.radius[is]) / dist;
     const int ct = T.count[it], cs = S.count[is];
     if (sep < theta) return (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);  // :155-179
+    if constexpr (DIM == 3 && MAC) {
+        // the learned MAC may accept what the theta MAC would refine
+        const int c = mac_accept(it, is, mac, mw, thr);
+        if (c >= 0) return c;
+    }
     // :180-203
     const bool lt = T.child0[it] < 0, lsf = S.child0[is] < 0;
     if (lt && lsf) return 0;
@@ -61,13 +66,13 @@ __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, 
     return ct > cs ? 4 : 5;
 }
 
-template <int DIM>
+template <int DIM, bool MAC>
 __global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__ cur, TreeView T, TreeView S,
-                                                 double theta, int thr, unsigned char *__restrict__ code,
+                                                 double theta, int thr, MacView mac, unsigned char *__restrict__ code,
                                                  long long *__restrict__ val) {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= G) return;
-    const int c = classify_pair<DIM>(cur[i].x, cur[i].y, T, S, theta, thr);
+    const int c = classify_pair<DIM, MAC>(cur[i].x, cur[i].y, T, S, theta, thr, mac, mac.w);
     code[i] = (unsigned char)c;
     val[i] = c < 4 ? 1ll : (1ll << 32);
 }
@@ -104,13 +109,18 @@ __device__ __forceinline__ void write_pair(int c, int it, int is, long long ex, 
 constexpr int FB = 1024;
 constexpr int kFuseMax = 8192;
 
-template <int DIM>
+template <int DIM, bool MAC>
 __global__ void __launch_bounds__(FB)
-    k_generation(int G, const int2 *__restrict__ cur, TreeView T, TreeView S, double theta, int thr, int lbase,
-                 btk_interact_pair *__restrict__ out, int2 *__restrict__ next, long long *tot) {
+    k_generation(int G, const int2 *__restrict__ cur, TreeView T, TreeView S, double theta, int thr, MacView mac,
+                 int lbase, btk_interact_pair *__restrict__ out, int2 *__restrict__ next, long long *tot) {
     __shared__ long long wsum[FB / 64];
     __shared__ long long btot;
+    __shared__ float mw[DIM == 3 && MAC ? kMacParams : 1];  // the learned MAC's weights
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    if constexpr (DIM == 3 && MAC) {
+        for (int i = tid; i < kMacParams; i += FB) mw[i] = mac.w[i];
+        __syncthreads();
+    }
     long long carry = 0;
     for (int c0 = 0; c0 < G; c0 += FB) {
         const int i = c0 + tid;
@@ -119,7 +129,7 @@ __global__ void __launch_bounds__(FB)
         if (i < G) {
             it = cur[i].x;
             is = cur[i].y;
-            c = classify_pair<DIM>(it, is, T, S, theta, thr);
+            c = classify_pair<DIM, MAC>(it, is, T, S, theta, thr, mac, mw);
             v = c < 4 ? 1ll : (1ll << 32);
         }
         long long incl = v;
@@ -203,7 +213,8 @@ __global__ void k_type_count(int L, const btk_interact_pair *__restrict__ pairs,
 
 }  // namespace
 
-std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree &ts, hipStream_t s) {
+std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree &ts, hipStream_t s, bool learned_mac,
+                                 std::vector<int2> *visited) {
     BTK_REQUIRE(tt.dim == ts.dim, "target and source trees differ in dimension");
     const int dim = tt.dim;
     const int NC = dim == 3 ? 8 : 4;
@@ -211,6 +222,7 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
     l->dim = dim;
     const TreeView T = view(tt), S = view(ts);
     long long *hv = host_scratch().v;
+    const MacView mac = learned_mac ? mac_view(*cfg, tt, ts) : MacView{};
 
     // generation 0: the root pair of every problem (one pair for one tree)
     BTK_REQUIRE(tt.nprob == ts.nprob, "target and source forests hold different numbers of problems");
@@ -230,29 +242,43 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
         lcap = ncap;
     };
     while (G > 0) {
+        if (visited) {  // training samples: every visited pair
+            const size_t v0 = visited->size();
+            visited->resize(v0 + G);
+            BTK_HIP(hipMemcpyAsync(visited->data() + v0, cur.p, sizeof(int2) * G, hipMemcpyDeviceToHost, s));
+            BTK_HIP(hipStreamSynchronize(s));
+        }
         long long tot;
         if (G <= kFuseMax) {
             // one launch; at most G pairs emitted and NC * G children
             reserve((size_t)L + G);
             nxt.resize((size_t)NC * G);
-            if (dim == 3)
-                k_generation<3><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, L,
-                                                 l->pairs.p, nxt.p, hv);
+#define BTK_GEN(D, M)                                                                                      \
+    k_generation<D, M><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, mac, L, l->pairs.p, \
+                                        nxt.p, hv)
+            if (dim == 3 && mac.w)
+                BTK_GEN(3, true);
+            else if (dim == 3)
+                BTK_GEN(3, false);
             else
-                k_generation<2><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, L,
-                                                 l->pairs.p, nxt.p, hv);
+                BTK_GEN(2, false);
+#undef BTK_GEN
             BTK_LAUNCH_CHECK();
             BTK_HIP(hipStreamSynchronize(s));
             tot = hv[0];
         } else {
             DBuf<unsigned char> code(G);
             DBuf<long long> val(G), inc(G);
-            if (dim == 3)
-                k_classify<3><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh,
-                                                     code.p, val.p);
+#define BTK_CLS(D, M)                                                                                     \
+    k_classify<D, M><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, mac, code.p, \
+                                            val.p)
+            if (dim == 3 && mac.w)
+                BTK_CLS(3, true);
+            else if (dim == 3)
+                BTK_CLS(3, false);
             else
-                k_classify<2><<<nblk(G), TB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh,
-                                                     code.p, val.p);
+                BTK_CLS(2, false);
+#undef BTK_CLS
             BTK_LAUNCH_CHECK();
             inclusive_sum_i64(val.p, inc.p, G, s);
             BTK_HIP(hipMemcpyAsync(hv, inc.p + (G - 1), sizeof(long long), hipMemcpyDeviceToHost, s));

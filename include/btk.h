@@ -103,7 +103,8 @@ typedef struct btk_options {
     int exclude_self; /* 1 = skip pairs at zero distance; 0 = include them (reference) */
     int deepest_only; /* 1 = leaf_to_point on the deepest level only (downward_pass.cpp:26-28) */
     int precision;    /* BTK_PREC_* */
-    int reserved[3];
+    int mac;          /* 0 = the reference's fixed theta MAC; 1 = the learned MAC (btk_mac_*) where it accepts */
+    int reserved[2];
 } btk_options;
 
 /* Number of output components of a kernel: 1, 3, or 1 + dim. */
@@ -246,6 +247,49 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
                     const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
                     const double *d_q, int nrhs, int coincident, double *d_out, btk_solve_stats *stats,
                     void *stream);
+
+/* Learned MAC (no reference counterpart).  A network of 10 -> 64 -> 64 -> 12
+ * (ReLU; FP32; BTK_MAC_WEIGHTS parameters: W1[64][10], b1[64], W2[64][64],
+ * b2[64], W3[12][64], b3[12]) reads 10 features of a candidate cluster pair
+ * (separation ratio (r_t + r_s)/d, r_t/d, r_s/d, log2 of both point counts,
+ * both levels, log2(r_t/r_s), g/d and (r_t + r_s)/g with g the distance
+ * between the boxes) and predicts log10 of the relative error of its PC, CP
+ * and CC approximations at degrees 4, 6, 8 and 10 (output 4*type' + degree
+ * index, type' = PC 0, CP 1, CC 2; scaled by 1/4, clamped to [-14, 0]).  With opts->mac = 1 (3D only, degree 4, 6,
+ * 8 or 10) the traversal evaluates it on every pair that the theta MAC
+ * would refine whose boxes are apart (a positive gap) with separation ratio
+ * below 1.2, and emits the pair at its count-given type when the predicted
+ * error is within the budget and the approximation costs fewer evaluations
+ * than the direct sum of the pair; every other pair follows the theta MAC,
+ * which stays the fallback.  The model and budget are process-wide. */
+#define BTK_MAC_FEATURES 10
+#define BTK_MAC_OUTPUTS 12
+#define BTK_MAC_WEIGHTS 5644
+/* Training samples from the treecode's own traversal of the given 3D points:
+ * n_samples pairs drawn from the pairs the traversal visits whose boxes are
+ * apart with separation ratio in [0.3, 1.2), each with its features
+ * d_X[n][10] and the measured errors
+ * d_Y[n][12] of its approximations against the direct sum over up to 64
+ * target and 1024 source points of the pair (charges U[-1,1] from seed).
+ * d_pairs[n][2] (may be null) receives each sample's (target, source) panel
+ * ids.  *count receives the number of samples written (<= n_samples). */
+int btk_mac_sample(const btk_run_config *cfg, int n, const double *d_x, const double *d_y, const double *d_z,
+                   int n_samples, unsigned seed, float *d_X, float *d_Y, int *d_pairs, int *count, void *stream);
+/* Adam on the mean squared error of the scaled outputs, minibatches of
+ * `batch`, from the installed weights (random He initialisation from seed
+ * when none are installed or reinit != 0); installs the result; *loss gets
+ * the last epoch's mean loss. */
+int btk_mac_train(const float *d_X, const float *d_Y, int n, int epochs, int batch, float lr, unsigned seed,
+                  int reinit, float *loss, void *stream);
+int btk_mac_set_weights(const float *h_weights, int count);
+int btk_mac_get_weights(float *h_weights, int count);
+/* Maximum predicted relative error of an accepted pair (default 1e-7). */
+int btk_mac_set_budget(double rel_err);
+/* The installed network on d_X[n][10]: d_out[n][12] (scaled outputs). */
+int btk_mac_forward(const float *d_X, int n, float *d_out, void *stream);
+/* Loss and its gradient (BTK_MAC_WEIGHTS values, the weights' layout) for
+ * the installed weights on one batch. */
+int btk_mac_gradients(const float *d_X, const float *d_Y, int n, float *d_grad, float *loss, void *stream);
 
 /* A solve plan for one geometry (no reference counterpart; for iterative
  * methods that apply one operator to many vectors): the trees, the list and
