@@ -70,7 +70,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", choices=["both", "plain", "mixed"], default="both")
+    ap.add_argument("--variant", choices=["both", "plain", "mixed", "learned"], default="both",
+                    help="both = plain and mixed, and in 3D also mixed with the learned MAC")
     ap.add_argument("--no-degree-scan", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
     ap.add_argument("--n", type=int, default=None)
@@ -85,6 +86,35 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scale for the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+# per-pair error budgets of the learned MAC, tried from the largest down
+MAC_BUDGETS = (1e-5, 3e-6, 1e-6, 3e-7)
+
+
+def train_mac(btk, torch, wl, degree, leaf, theta):
+    """Learned MAC for 3D Plummer-like sets: samples from the treecode's own
+    runs on three 1M-point Plummer sets (a = 0.3, other seeds than the
+    workload's), 10000 pairs each, then 200 epochs of Adam; returns the
+    training record."""
+    cfg = btk.make_config(3, degree, leaf, theta, 0.0)
+    Xs, Ys, total = [], [], 0
+    t0 = time.perf_counter()
+    for seed in (11, 12, 13):
+        x, y, z, _ = wl.plummer_3d(1_000_000, seed, a=0.3, rmax=10.0)
+        d = [torch.from_numpy(a).cuda() for a in (x, y, z)]
+        X = torch.zeros(10000 * 10, dtype=torch.float32, device="cuda")
+        Y = torch.zeros(10000 * 12, dtype=torch.float32, device="cuda")
+        m = btk.mac_sample(cfg, d[0], d[1], d[2], 10000, seed, X, Y)
+        Xs.append(X[:m * 10])
+        Ys.append(Y[:m * 12])
+        total += m
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss = btk.mac_train(torch.cat(Xs), torch.cat(Ys), total, epochs=200, batch=1024, lr=2e-3, seed=3, reinit=True)
+    torch.cuda.synchronize()
+    return {"samples": total, "sample_s": t1 - t0, "train_s": time.perf_counter() - t1, "loss": loss,
+            "sets": "3 Plummer sets of 1M points (a=0.3, seeds 11-13)"}
 
 
 def log(msg):
@@ -387,8 +417,10 @@ def main():
     C = btk.components(dim, kernel)
     fpe = FLOPS[(dim, kernel)]
     variants = ["plain", "mixed"] if args.variant == "both" else [args.variant]
-    prec = {"plain": btk.PREC_FP64, "mixed": btk.PREC_MIXED}
-    dtype = {"plain": "f64", "mixed": "f32"}
+    if args.variant == "both" and dim == 3:
+        variants.append("learned")
+    prec = {"plain": btk.PREC_FP64, "mixed": btk.PREC_MIXED, "learned": btk.PREC_MIXED}
+    dtype = {"plain": "f64", "mixed": "f32", "learned": "f32"}
 
     # FP64 direct sum (GPU; checked against the oracle in the tests): every
     # target up to 1M, 10k sampled targets above
@@ -408,22 +440,44 @@ def main():
     res = {}
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
-        opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v])
+        opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"))
+        train = None
+        if v == "learned":  # untimed, like the warmup
+            log("learned MAC: samples and training")
+            train = train_mac(btk, torch, wl, degree, leaf, theta)
         btk.set_tuning()  # defaults
         if args.shapes:
             btk.set_tuning(*[int(x) for x in args.shapes.split(",")])
         elif not args.no_autotune:  # untimed, like the warmup
             log(f"autotune {v}")
+            if v == "learned":
+                btk.mac_set_budget(MAC_BUDGETS[0])
             btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
         t = btk.get_tuning()
-        log(f"timing {v}")
-        ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
-        res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
+        if v == "learned":
+            # the largest per-pair budget whose result stays within 1e-6
+            scan = []
+            for b in MAC_BUDGETS:
+                btk.mac_set_budget(b)
+                log(f"timing learned, budget {b:g}")
+                ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
+                r = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
+                del out
+                scan.append({"budget": b, "ms_per_step": ms, "rel_l2": r.get("rel_l2")})
+                if ds is None or r["rel_l2"] <= 1e-6:
+                    break
+            res[v] = dict(r, mac_budget=b, mac_budget_scan=scan, mac_training=train)
+        else:
+            log(f"timing {v}")
+            ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
+            res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
+            del out
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
                                    "far_tpt": t.far_tpt, "mutual": t.mutual,
                                    "autotuned": not (args.no_autotune or args.shapes)}
-        del out
     head = "mixed" if "mixed" in res else variants[0]
+    if "learned" in res and (ds is None or res["learned"]["rel_l2"] <= 1e-6):
+        head = "learned"  # the fastest variant that meets the metric's error
     main_r = res[head]
 
     # the same workload at the smallest degree whose error reaches the metric's 1e-6
@@ -432,7 +486,8 @@ def main():
         for deg in range(degree, 11):
             log(f"degree scan: degree {deg}")
             cfg = btk.make_config(dim, deg, leaf, theta, 0.0)
-            opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[head])
+            opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[head],
+                               mac=int(head == "learned"))
             ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, 1)
             r = summarize(stats, ms, n, dtype[head], out, ds, fpe, idx)
             del out
@@ -474,6 +529,8 @@ def main():
                                 traffic_source=os.path.relpath(tfile, HERE))
     if "plain" in res and "mixed" in res:
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
+    if "plain" in res and "learned" in res:
+        line["learned_over_plain"] = res["learned"]["targets_per_s"] / res["plain"]["targets_per_s"]
     # above 1M points the oracle's tree, traversal and far field alone take
     # minutes, so the CPU leg runs there only when asked for
     if not args.no_cpu_baseline and (n <= 1_000_000 or args.cpu_baseline):
