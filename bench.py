@@ -43,9 +43,9 @@ WORKLOADS = {
     "3d-sphere-plummer-4M": (3, 4_000_000, "sphere3", 8, 0.7, 2000, 2,
                              "3D potential+gradient, N=M=4,000,000 (half unit sphere, half Plummer a=0.3 r<=2), "
                              "q~U[-1,1], degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
-    "3d-plummer-16M": (3, 16_000_000, "plummer3", 8, 0.7, 2000, 2,
+    "3d-plummer-16M": (3, 16_000_000, "plummer3", 8, 0.7, 1000, 2,
                        "3D potential+gradient, N=M=16,000,000 Plummer a=0.3 truncated at r<=10, q~U[-1,1], "
-                       "degree 8, theta 0.7 (the fixed MAC), leaf 2000, eps 0, self excluded"),
+                       "degree 8, theta 0.7 (the fixed MAC), leaf 1000, eps 0, self excluded"),
 }
 
 
@@ -89,7 +89,7 @@ def parse():
 
 
 # per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (1e-5, 3e-6, 1e-6, 3e-7)
+MAC_BUDGETS = (1e-5, 6e-6, 4e-6, 3e-6, 2e-6, 1e-6, 3e-7)
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta):
