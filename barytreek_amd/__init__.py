@@ -30,7 +30,8 @@ EXPORTED = [
     "btk_tree_construction_2d", "btk_tree_construction_3d", "btk_tree_get_info",
     "btk_tree_copy_panels_2d", "btk_tree_copy_panels_3d", "btk_tree_copy_point_leaf_panel",
     "btk_tree_copy_panel_points_inside", "btk_tree_copy_permutation", "btk_tree_destroy",
-    "btk_dual_tree_traversal_2d", "btk_dual_tree_traversal_3d", "btk_list_count", "btk_list_copy",
+    "btk_dual_tree_traversal_2d", "btk_dual_tree_traversal_3d", "btk_dual_tree_traversal_learned_3d",
+    "btk_list_count", "btk_list_copy",
     "btk_split_interactions", "btk_list_destroy", "btk_upward_pass_2d", "btk_upward_pass_3d",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
@@ -128,6 +129,7 @@ def lib():
     L.btk_tree_destroy.restype = None
     L.btk_dual_tree_traversal_2d.argtypes = [cfgp, vp, vp, vp, C.POINTER(vp)]
     L.btk_dual_tree_traversal_3d.argtypes = [cfgp, vp, vp, vp, C.POINTER(vp)]
+    L.btk_dual_tree_traversal_learned_3d.argtypes = [cfgp, vp, vp, vp, C.POINTER(vp)]
     L.btk_list_count.argtypes = [vp]
     L.btk_list_copy.argtypes = [vp, vp]
     L.btk_split_interactions.argtypes = [vp, vp, vp, vp, vp]
@@ -283,11 +285,16 @@ class Tree:
 
 
 class InteractionList:
-    """A device-resident interaction list (btk_list handle)."""
+    """A device-resident interaction list (btk_list handle).  learned_mac:
+    the 3D traversal with the installed learned MAC, whose pairs carry their
+    degree index in bits 2-3 of interact_type (degree 4 + 2 * index)."""
 
-    def __init__(self, cfg: RunConfig, target: Tree, source: Tree, stream=None):
+    def __init__(self, cfg: RunConfig, target: Tree, source: Tree, stream=None, learned_mac=False):
         h = C.c_void_p()
-        f = lib().btk_dual_tree_traversal_3d if cfg.dim == 3 else lib().btk_dual_tree_traversal_2d
+        if learned_mac:
+            f = lib().btk_dual_tree_traversal_learned_3d
+        else:
+            f = lib().btk_dual_tree_traversal_3d if cfg.dim == 3 else lib().btk_dual_tree_traversal_2d
         _check(f(C.byref(cfg), target.h, source.h, _stream(stream), C.byref(h)))
         self.h = h
         self.count = lib().btk_list_count(self.h)

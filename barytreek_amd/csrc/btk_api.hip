@@ -18,7 +18,7 @@ struct btk_list {
 struct btk_plan {
     btk_run_config cfg{};
     btk_options o{};
-    int C = 1, P = 1, m = 0, n = 0;
+    int C = 1, m = 0, n = 0;
     std::unique_ptr<btk::Tree> ts, tt_own;
     const btk::Tree *tt = nullptr;
     std::unique_ptr<btk::List> l;
@@ -329,13 +329,13 @@ void btk_tree_destroy(btk_tree *tree) {
 }
 
 static int traversal(btk_run_config *cfg, int dim, const btk_tree *target, const btk_tree *source, void *stream,
-                     btk_list **list) {
+                     btk_list **list, bool learned_mac = false) {
     return guarded([&] {
         check_cfg(cfg, dim);
         BTK_REQUIRE(target && source && list, "null argument");
         BTK_REQUIRE(target->t->dim == dim && source->t->dim == dim, "tree dimension mismatch");
         auto h = std::make_unique<btk_list>();
-        h->l = btk::build_list(cfg, *target->t, *source->t, as_stream(stream));
+        h->l = btk::build_list(cfg, *target->t, *source->t, as_stream(stream), learned_mac);
         *list = h.release();
     });
 }
@@ -348,6 +348,11 @@ int btk_dual_tree_traversal_2d(btk_run_config *cfg, const btk_tree *target, cons
 int btk_dual_tree_traversal_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
                                btk_list **list) {
     return traversal(cfg, 3, target, source, stream, list);
+}
+
+int btk_dual_tree_traversal_learned_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source,
+                                       void *stream, btk_list **list) {
+    return traversal(cfg, 3, target, source, stream, list, true);
 }
 
 int btk_list_count(const btk_list *list) { return list ? list->l->count : -1; }
@@ -375,9 +380,10 @@ int btk_split_interactions(const btk_list *list, btk_interact_pair *pp, btk_inte
         btk_interact_pair *dst[4] = {pp, pc, cp, cc};
         long long at[4] = {0, 0, 0, 0};
         for (const auto &p : all) {
-            BTK_REQUIRE(p.interact_type >= 0 && p.interact_type < 4, "bad interaction type");
-            if (dst[p.interact_type]) dst[p.interact_type][at[p.interact_type]] = p;
-            at[p.interact_type]++;
+            BTK_REQUIRE(p.interact_type >= 0 && p.interact_type < 16, "bad interaction type");
+            const int ty = p.interact_type & 3;  // bits 2-3: the degree index of a learned-MAC list
+            if (dst[ty]) dst[ty][at[ty]] = p;
+            at[ty]++;
         }
     });
 }
@@ -420,6 +426,8 @@ static int interactions(const btk_run_config *cfg, int dim, const btk_options *o
         BTK_REQUIRE(target->t->dim == dim && source->t->dim == dim, "tree dimension mismatch");
         const btk_options o = opts_or_default(opts);
         hipStream_t s = as_stream(stream);
+        BTK_REQUIRE(list->l->ndeg == 1,
+                    "a learned-MAC list mixes interpolation degrees; its interactions run inside btk_solve");
         const btk::Tree &ts = *source->t;
         btk::DBuf<double> q(ts.n ? ts.n : 1);
         btk::gather_sorted(ts, d_charges, q.p, s);
@@ -504,11 +512,11 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
         auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
-        const int P = btk::ipow(cfg.interp_degree + 1, dim);
         const int C = btk::components(dim, o.kernel);
-        btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P);
+        btk::DBuf<double> q(n ? n : 1), W(btk::proxy_offset(*l, ts->pc, 1, l->ndeg)),
+            Psi(btk::proxy_offset(*l, tt.pc, C, l->ndeg));
         btk::gather_sorted(*ts, d_q, q.p, s);
-        btk::upward_pass(cfg, *ts, q.p, W.p, s);
+        btk::upward_all(cfg, *l, *ts, q.p, W.p, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
@@ -521,7 +529,7 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
         if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
         if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
-        btk::downward_pass(cfg, o, tt, Psi.p, d_out, s);
+        btk::downward_all(cfg, o, *l, tt, Psi.p, d_out, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
         if (stats) {
@@ -561,11 +569,11 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
         if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
         const btk::Tree &tt = coincident ? *ts : *tt_own;
         auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
-        const int P = btk::ipow(cfg.interp_degree + 1, dim);
         const int C = btk::components(dim, o.kernel);
-        btk::DBuf<double> q(n ? n : 1), W((size_t)ts->pc * P), Psi((size_t)C * tt.pc * P), out((size_t)C * (m ? m : 1));
+        btk::DBuf<double> q(n ? n : 1), W(btk::proxy_offset(*l, ts->pc, 1, l->ndeg)),
+            Psi(btk::proxy_offset(*l, tt.pc, C, l->ndeg)), out((size_t)C * (m ? m : 1));
         btk::gather_sorted(*ts, d_q, q.p, s);
-        btk::upward_pass(cfg, *ts, q.p, W.p, s);
+        btk::upward_all(cfg, *l, *ts, q.p, W.p, s);
         // Candidate shapes: work-item targets x targets per lane x mutual
         // (the last two in mixed precision only).  The near and far launches
         // are independent, so call i tries near shape i and far shape i.
@@ -702,11 +710,10 @@ int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m
         p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s);
         p->m = m;
         p->n = n;
-        p->P = btk::ipow(p->cfg.interp_degree + 1, dim);
         p->C = btk::components(dim, p->o.kernel);
         p->q.resize(n ? n : 1);
-        p->W.resize((size_t)p->ts->pc * p->P);
-        p->Psi.resize((size_t)p->C * p->tt->pc * p->P);
+        p->W.resize(btk::proxy_offset(*p->l, p->ts->pc, 1, p->l->ndeg));
+        p->Psi.resize(btk::proxy_offset(*p->l, p->tt->pc, p->C, p->l->ndeg));
         BTK_HIP(hipStreamCreateWithFlags(&p->cs, hipStreamNonBlocking));
         BTK_HIP(hipStreamSynchronize(s));
         *plan = p.release();
@@ -723,10 +730,10 @@ int btk_plan_execute(btk_plan *p, const double *d_q, double *d_out, void *stream
             BTK_HIP(hipStreamBeginCapture(p->cs, hipStreamCaptureModeThreadLocal));
             try {
                 btk::gather_sorted(*p->ts, d_q, p->q.p, p->cs);
-                btk::upward_pass(p->cfg, *p->ts, p->q.p, p->W.p, p->cs);
+                btk::upward_all(p->cfg, *p->l, *p->ts, p->q.p, p->W.p, p->cs);
                 btk::zero_fill(p->Psi.p, p->Psi.n, p->cs);
                 btk::launch_interactions(*p->ip, p->q.p, p->W.p, d_out, p->Psi.p, p->cs);
-                btk::downward_pass(p->cfg, p->o, *p->tt, p->Psi.p, d_out, p->cs);
+                btk::downward_all(p->cfg, p->o, *p->l, *p->tt, p->Psi.p, d_out, p->cs);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(p->cs, &g);
@@ -774,12 +781,12 @@ int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int n
         auto l = btk::build_list(&cfg, *t, *t, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
-        const int P = btk::ipow(cfg.interp_degree + 1, dim);
         const int C = btk::components(dim, o.kernel);
         const int n = t->n;
-        btk::DBuf<double> q(n ? n : 1), W((size_t)t->pc * P), Psi((size_t)C * t->pc * P);
+        btk::DBuf<double> q(n ? n : 1), W(btk::proxy_offset(*l, t->pc, 1, l->ndeg)),
+            Psi(btk::proxy_offset(*l, t->pc, C, l->ndeg));
         btk::gather_sorted(*t, d_q, q.p, s);
-        btk::upward_pass(cfg, *t, q.p, W.p, s);
+        btk::upward_all(cfg, *l, *t, q.p, W.p, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
@@ -792,7 +799,7 @@ int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int n
         if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
         if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
-        btk::downward_pass(cfg, o, *t, Psi.p, d_out, s);
+        btk::downward_all(cfg, o, *l, *t, Psi.p, d_out, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
         if (stats) {
@@ -840,20 +847,19 @@ int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m
         auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
-        const int P = btk::ipow(cfg.interp_degree + 1, dim);
         const int C = btk::components(dim, o.kernel);
-        const size_t wrows = (size_t)ts->pc * P, prow = (size_t)C * tt.pc * P;
+        const size_t wrows = btk::proxy_offset(*l, ts->pc, 1, l->ndeg), prow = btk::proxy_offset(*l, tt.pc, C, l->ndeg);
         btk::DBuf<double> q((size_t)nrhs * (n ? n : 1)), W((size_t)nrhs * wrows), Psi((size_t)nrhs * prow);
         for (int r = 0; r < nrhs; r++) {
             btk::gather_sorted(*ts, d_q + (size_t)r * n, q.p + (size_t)r * n, s);
-            btk::upward_pass(cfg, *ts, q.p + (size_t)r * n, W.p + (size_t)r * wrows, s);
+            btk::upward_all(cfg, *l, *ts, q.p + (size_t)r * n, W.p + (size_t)r * wrows, s);
         }
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
         btk::KernelEvents kev;
         for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
-        if (o.precision == BTK_PREC_MIXED && nrhs >= 16) {
+        if (o.precision == BTK_PREC_MIXED && nrhs >= 16 && l->ndeg == 1) {
             const int kp = (nrhs + 63) / 64 * 64;
             btk::DBuf<float> qs((size_t)(n ? n : 1) * kp), wt(wrows * kp);
             btk::gather_multi_f32(*ts, d_q, nrhs, kp, qs.p, s);
@@ -872,7 +878,7 @@ int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m
         if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
         for (int r = 0; r < nrhs; r++)
-            btk::downward_pass(cfg, o, tt, Psi.p + (size_t)r * prow, d_out + (size_t)r * C * m, s);
+            btk::downward_all(cfg, o, *l, tt, Psi.p + (size_t)r * prow, d_out + (size_t)r * C * m, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
         if (stats) {

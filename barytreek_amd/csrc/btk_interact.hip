@@ -16,11 +16,11 @@ inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
 // Work items per target panel: blocks of `per` targets times entry slices.
 // With meta, entries whose source count is zero (taken by another kernel)
 // are not counted, and a panel left without entries gets no work item.
-__global__ void k_task_count(int pct, const int *__restrict__ goff, int far, const int *__restrict__ tcount, int P,
-                             int per, int nsplit, const int4 *__restrict__ meta, int *__restrict__ ntask) {
+__global__ void k_task_count(int pct, const int *__restrict__ goff, int grp, int far, const int *__restrict__ tcount,
+                             int P, int per, int nsplit, const int4 *__restrict__ meta, int *__restrict__ ntask) {
     const int p = blockIdx.x * TB + threadIdx.x;
     if (p >= pct) return;
-    const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
+    const int g0 = goff[grp * pct + p], g1 = goff[grp * pct + p + 1];
     int ne = g1 - g0;
     if (meta) {
         int nz = 0;
@@ -32,14 +32,16 @@ __global__ void k_task_count(int pct, const int *__restrict__ goff, int far, con
     ntask[p] = nb * min(nsplit, ne);
 }
 
-// Source descriptor of every grouped entry (EntryView).
+// Source descriptor of every grouped entry (EntryView); a cluster source has
+// Pd[degree index] proxies.
 __global__ void k_entry_meta(int n, const int *__restrict__ gsrc, const unsigned char *__restrict__ gtype,
-                             const int *__restrict__ start, const int *__restrict__ count, int P,
+                             const int *__restrict__ start, const int *__restrict__ count, int4 Pd,
                              int4 *__restrict__ meta) {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const int S = gsrc[i];
-    const int ty = gtype[i];
+    const int ty = gtype[i], di = ty >> 2;
+    const int P = di == 0 ? Pd.x : (di == 1 ? Pd.y : (di == 2 ? Pd.z : Pd.w));
     meta[i] = (ty & 1) ? make_int4(S, 0, P, ty) : make_int4(S, start[S], count[S], ty);
 }
 
@@ -63,7 +65,7 @@ __device__ __forceinline__ long long wave_incl_scan(long long v) {
 }
 
 __global__ void __launch_bounds__(kLanes)
-    k_task_fill(int pct, const int *__restrict__ goff, int far, const int *__restrict__ gsrc,
+    k_task_fill(int pct, const int *__restrict__ goff, int grp, int far, const int *__restrict__ gsrc,
                 const unsigned char *__restrict__ gtype, const int *__restrict__ scount, int P, int per, int nsplit,
                 const int4 *__restrict__ meta, const int *__restrict__ tcount, const int *__restrict__ ntask,
                 const int *__restrict__ tbase, int *__restrict__ tpanel, int *__restrict__ tfirst,
@@ -74,7 +76,7 @@ __global__ void __launch_bounds__(kLanes)
     const int p = blockIdx.x;
     const int lane = threadIdx.x;
     if (ntask[p] == 0) return;
-    const int g0 = goff[far * pct + p], g1 = goff[far * pct + p + 1];
+    const int g0 = goff[grp * pct + p], g1 = goff[grp * pct + p + 1];
     const int ne_all = g1 - g0;
     auto nsrc = [&](int e) -> long long { return meta ? meta[e].z : ((gtype[e] & 1) ? P : scount[gsrc[e]]); };
     long long carry = 0;
@@ -137,36 +139,40 @@ template <int DIM, int KERNEL>
 void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
                 hipStream_t s, KernelEvents *kev) {
     const Tree &tt = *p.tt, &ts = *p.ts;
-    const int n1 = p.cfg.interp_degree + 1;
-    const int P = ipow(n1, DIM);
-    const SrcView sv{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
-                     ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W};
     for (int far = 0; far < 2; far++) {
-        const InteractPlan::Group &g = p.g[far];
-        if (!g.present) continue;
-        double *dst = far ? Psi : out;
-        InteractLaunch L;
-        L.tk = TaskView{g.tk.panel.p, g.tk.first.p, g.tk.e0.p, g.tk.e1.p};
-        L.ev = EntryView{p.meta.p};
-        L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
-                       tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
-        L.sv = sv;
-        L.n_tasks = g.tk.n;
-        L.n1 = n1;
-        L.tpt = g.tpt;
-        L.far = far;
-        L.excl = p.o.exclude_self ? 1 : 0;
-        L.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
-        L.atomic = g.nsplit > 1 ? 1 : 0;
-        // far targets are proxy points, which no zero-distance rule ties to a panel
-        L.same_tree = (&tt == &ts && !far) ? 1 : 0;
-        L.tabs = p.tabs;
-        L.kp = p.kp;
-        L.out = dst;
-        L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
+        bool any = false;
+        for (int di = 0; di < p.ndeg; di++) any |= p.g[far][di].present;
+        if (!any) continue;
         if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
-        if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, s);
-        launch_interact<DIM, KERNEL>(L, s);
+        for (int di = 0; di < p.ndeg; di++) {
+            const InteractPlan::Group &g = p.g[far][di];
+            if (!g.present) continue;
+            const int n1 = p.deg[di] + 1;
+            const int P = ipow(n1, DIM);
+            double *dst = far ? Psi + p.poff[di] : out;
+            InteractLaunch L;
+            L.tk = TaskView{g.tk.panel.p, g.tk.first.p, g.tk.e0.p, g.tk.e1.p};
+            L.ev = EntryView{p.meta.p};
+            L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
+                           tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
+            L.sv = SrcView{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
+                           ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W + p.woff[di]};
+            L.n_tasks = g.tk.n;
+            L.n1 = n1;
+            L.tpt = g.tpt;
+            L.far = far;
+            L.excl = p.o.exclude_self ? 1 : 0;
+            L.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
+            L.atomic = g.nsplit > 1 ? 1 : 0;
+            // far targets are proxy points, which no zero-distance rule ties to a panel
+            L.same_tree = (&tt == &ts && !far) ? 1 : 0;
+            L.tabs = p.tabs[di];
+            L.kp = p.kp;
+            L.out = dst;
+            L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
+            if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, s);
+            launch_interact<DIM, KERNEL>(L, s);
+        }
         if (kev) {
             BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
             (far ? kev->far_done : kev->near_done) = true;
@@ -227,24 +233,34 @@ void launch_direct(const KParams &kp, int m, const int *idx, const double *xt, c
 
 }  // namespace
 
-void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
-                 hipStream_t s, const int4 *meta) {
+// Work items per panel of group grp (nt) and their exclusive sum (tb);
+// returns the total.
+int count_tasks(const List &l, const Tree &tt, int grp, int P, int per, int nsplit, const int4 *meta, DBuf<int> &nt,
+                DBuf<int> &tb, hipStream_t s) {
     const int pct = tt.pc;
-    DBuf<int> nt(pct), tb(pct);
-    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, far, tt.count.p, P, per, nsplit, meta, nt.p);
+    const int far = grp >= l.ndeg ? 1 : 0;
+    k_task_count<<<nblk(pct), TB, 0, s>>>(pct, l.goff.p, grp, far, tt.count.p, P, per, nsplit, meta, nt.p);
     BTK_LAUNCH_CHECK();
     exclusive_sum_i32(nt.p, tb.p, pct, s);
     int *h = reinterpret_cast<int *>(host_scratch().v);
     BTK_HIP(hipMemcpyAsync(h, tb.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
     BTK_HIP(hipMemcpyAsync(h + 1, nt.p + (pct - 1), sizeof(int), hipMemcpyDeviceToHost, s));
     BTK_HIP(hipStreamSynchronize(s));
-    tk.n = h[0] + h[1];
+    return h[0] + h[1];
+}
+
+void build_tasks(const List &l, const Tree &tt, const Tree &ts, int grp, int P, int per, int nsplit, Tasks &tk,
+                 hipStream_t s, const int4 *meta) {
+    const int pct = tt.pc;
+    const int far = grp >= l.ndeg ? 1 : 0;
+    DBuf<int> nt(pct), tb(pct);
+    tk.n = count_tasks(l, tt, grp, P, per, nsplit, meta, nt, tb, s);
     const size_t m = tk.n ? tk.n : 1;
     tk.panel.resize(m);
     tk.first.resize(m);
     tk.e0.resize(m);
     tk.e1.resize(m);
-    k_task_fill<<<pct, kLanes, 0, s>>>(pct, l.goff.p, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit, meta,
+    k_task_fill<<<pct, kLanes, 0, s>>>(pct, l.goff.p, grp, far, l.gsrc.p, l.gtype.p, ts.count.p, P, per, nsplit, meta,
                                        tt.count.p, nt.p, tb.p, tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p);
     BTK_LAUNCH_CHECK();
 }
@@ -268,40 +284,61 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
     BTK_REQUIRE(o.precision == BTK_PREC_FP64 || o.precision == BTK_PREC_MIXED, "unknown precision");
     BTK_REQUIRE(!(dim == 2 && o.kernel == BTK_KERNEL_REF_VEL3D), "the reference vel kernel is 3D only");
+    BTK_REQUIRE(l.ndeg >= 1 && l.ndeg <= 4 && l.deg[l.ndeg - 1] == cfg.interp_degree,
+                "the list was built for another interp_degree");
     auto p = std::make_shared<InteractPlan>();
     p->cfg = cfg;
     p->o = o;
     p->tt = &tt;
     p->ts = &ts;
     p->kp = make_kparams(dim, cfg.ker_eps, o);
-    p->tabs = node_tables(cfg.interp_degree);
-    const int P = ipow(n1, dim);
+    p->ndeg = l.ndeg;
+    const int C = components(dim, o.kernel);
+    int Pd[4] = {0, 0, 0, 0};
+    for (int di = 0; di < l.ndeg; di++) {
+        p->deg[di] = l.deg[di];
+        p->tabs[di] = node_tables(l.deg[di]);
+        p->woff[di] = proxy_offset(l, ts.pc, 1, di);
+        p->poff[di] = proxy_offset(l, tt.pc, C, di);
+        Pd[di] = ipow(l.deg[di] + 1, dim);
+    }
     const bool f32 = o.precision == BTK_PREC_MIXED;
     const btk_tuning &tu = tuning();
     p->meta.resize(l.count ? l.count : 1);
     if (l.count) {
-        k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p, P,
-                                                  p->meta.p);
+        k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p,
+                                                  make_int4(Pd[0], Pd[1], Pd[2], Pd[3]), p->meta.p);
         BTK_LAUNCH_CHECK();
     }
     for (int far = 0; far < 2; far++) {
-        InteractPlan::Group &g = p->g[far];
-        g.present = l.counts[2 * far] + l.counts[2 * far + 1] > 0;
-        if (!g.present) continue;
-        const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
-        // default: two targets per lane where the kernel is short (2D), one in 3D
-        g.tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((dim == 3 || (far && P <= kLanes)) ? 1 : 2));
-        const int per = kLanes * g.tpt;
-        const int n_clusters = tt.pc - tt.n_leaves;
-        const long long base = far ? (long long)n_clusters * ((P + per - 1) / per)
-                                   : (long long)tt.n / per + tt.n_leaves;
-        const long long items = far ? (tu.far_items ? tu.far_items : kFarItems)
-                                    : (tu.near_items ? tu.near_items : kNearItems);
-        g.nsplit = choose_split(base, items);
-        // mixed precision, one tree, when asked for: mutual PP pairs once for both directions
-        g.mutual = !far && f32 && &tt == &ts && tu.mutual;
-        if (g.mutual) build_mutual(l, tt, g.tpt, p->meta.p, g.mu, s);
-        build_tasks(l, tt, ts, far, P, per, g.nsplit, g.tk, s, g.mutual ? p->meta.p : nullptr);
+        if (l.counts[2 * far] + l.counts[2 * far + 1] == 0) continue;
+        for (int di = 0; di < l.ndeg; di++) {
+            InteractPlan::Group &g = p->g[far][di];
+            const int grp = far * l.ndeg + di;
+            const int P = Pd[di];
+            const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
+            // default: two targets per lane where the kernel is short (2D), one in 3D
+            g.tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((dim == 3 || (far && P <= kLanes)) ? 1 : 2));
+            const int per = kLanes * g.tpt;
+            // mixed precision, one tree, when asked for: mutual PP pairs once
+            // for both directions (PP entries are in the degree-0 near group)
+            g.mutual = !far && di == 0 && f32 && &tt == &ts && tu.mutual;
+            if (g.mutual) build_mutual(l, tt, g.tpt, p->meta.p, g.mu, s);
+            const int4 *meta = g.mutual ? p->meta.p : nullptr;
+            long long base;
+            if (l.ndeg == 1) {  // one degree: work items of the whole class, bounded from the tree
+                const int n_clusters = tt.pc - tt.n_leaves;
+                base = far ? (long long)n_clusters * ((P + per - 1) / per) : (long long)tt.n / per + tt.n_leaves;
+            } else {  // several: this group's own items at one slice per panel
+                DBuf<int> nt(tt.pc), tb(tt.pc);
+                base = count_tasks(l, tt, grp, P, per, 1, meta, nt, tb, s);
+            }
+            const long long items = far ? (tu.far_items ? tu.far_items : kFarItems)
+                                        : (tu.near_items ? tu.near_items : kNearItems);
+            g.nsplit = choose_split(base, items);
+            build_tasks(l, tt, ts, grp, P, per, g.nsplit, g.tk, s, meta);
+            g.present = g.tk.n > 0 || g.mutual;
+        }
     }
     return p;
 }

@@ -302,15 +302,16 @@ __global__ void __launch_bounds__(kLanes)
     }
 }
 
-// Work items of one group (near: PP+PC, far: CP+CC): blocks of `per`
-// targets of a panel times slices of its entries, balanced by source count.
+// Work items of one entry group of the list (near: PP+PC, far: CP+CC, at one
+// degree): blocks of `per` targets of a panel times slices of its entries,
+// balanced by source count.
 struct Tasks {
     int n = 0;
     DBuf<int> panel, first, e0, e1;
 };
 // With meta, entries whose source count meta[e].z is zero are left out of the
 // balance (another kernel evaluates them).
-void build_tasks(const List &l, const Tree &tt, const Tree &ts, int far, int P, int per, int nsplit, Tasks &tk,
+void build_tasks(const List &l, const Tree &tt, const Tree &ts, int grp, int P, int per, int nsplit, Tasks &tk,
                  hipStream_t s, const int4 *meta = nullptr);
 // Entry slices per panel so that the launch has about target_items work
 // items (at most 16 slices per panel).
@@ -353,7 +354,10 @@ struct InteractPlan {
     btk_options o{};
     const Tree *tt = nullptr, *ts = nullptr;
     KParams kp{};
-    NodeTables tabs{};
+    int ndeg = 1;
+    int deg[4] = {0, 0, 0, 0};
+    NodeTables tabs[4]{};  // per degree index
+    size_t woff[4]{}, poff[4]{};  // each degree's block in W and Psi (proxy_offset)
     DBuf<int4> meta;
     struct Group {
         bool present = false;
@@ -361,7 +365,7 @@ struct InteractPlan {
         bool mutual = false;
         Tasks tk;
         Mutual mu;
-    } g[2];
+    } g[2][4];  // [far][degree index]
 };
 
 // Launch of one interaction group with everything fixed but the kernel.

@@ -144,16 +144,28 @@ struct List {
     int count = 0;
     long long counts[4] = {0, 0, 0, 0};
     DBuf<btk_interact_pair> pairs;  // FIFO order
-    // Entries grouped by target panel, stable in FIFO order.  "Near" entries
-    // (PP, PC: the targets are points of a target leaf) of panel p are
-    // [goff[p], goff[p+1]); "far" entries (CP, CC: the targets are the proxy
-    // points of a target cluster) of panel p are [goff[pct+p], goff[pct+p+1]).
+    // Interpolation degrees in use, ascending: deg[ndeg - 1] is the
+    // configured one.  An entry's interact_type holds its type in bits 0-1
+    // and its degree index in bits 2-3; one degree unless the learned MAC
+    // chose per pair.
+    int ndeg = 1;
+    int deg[4] = {0, 0, 0, 0};
+    // Entries grouped by (near/far, degree index, target panel), stable in
+    // FIFO order; group g = far * ndeg + di.  "Near" entries (PP, PC: the
+    // targets are points of a target leaf) and "far" entries (CP, CC: the
+    // targets are the proxy points of a target cluster) of group g and panel
+    // p are [goff[g*pct+p], goff[g*pct+p+1]).  PP entries have degree index 0.
     int pct = 0;
-    DBuf<int> goff;             // [2*pct + 1]
+    DBuf<int> goff;             // [2*ndeg*pct + 1]
     DBuf<int> gsrc;             // source panel of each entry
     DBuf<unsigned char> gtype;  // interact_type of each entry
     double evals[4] = {0, 0, 0, 0};  // kernel evaluations per type
 };
+
+// Element offset of degree index di's block in a buffer that holds the proxy
+// arrays of every degree the list uses, `comps` arrays of pc * (deg+1)^dim
+// per degree; di = l.ndeg gives the buffer's size.
+size_t proxy_offset(const List &l, int pc, int comps, int di);
 
 // Interpolation tables (host-computed with libm, so that the nodes are the
 // same bits as the reference's cos() calls).
@@ -197,6 +209,8 @@ struct KernelEvents {
     hipEvent_t ev[4];
     bool near_done = false, far_done = false;
 };
+// W and Psi hold every degree of the list (proxy_offset with 1 and C
+// components); with one degree they are the plain arrays.
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
                       const List &l, const double *q_sorted, const double *W, double *out, double *Psi,
                       hipStream_t s, KernelEvents *kev = nullptr);
@@ -209,6 +223,17 @@ void launch_interactions(const InteractPlan &p, const double *q_sorted, const do
                          hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
                    hipStream_t s);
+// The passes over every degree of a list.  upward_all runs the upward pass at
+// the configured degree and restricts its charges to each lower degree, which
+// equals that degree's own upward pass: degree-d basis functions are
+// polynomials the degree-n interpolation reproduces.  downward_all extends
+// each lower degree's potentials into the configured degree's (the degree-d
+// interpolant evaluated at the degree-n proxies, reproduced by degree-n
+// interpolation in turn) and runs one downward pass.
+void upward_all(const btk_run_config &cfg, const List &l, const Tree &src, const double *q_sorted, double *W,
+                hipStream_t s);
+void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l, const Tree &tt, double *Psi,
+                  double *out, hipStream_t s);
 
 // Many charge vectors (btk_mrhs.hip).  q is [K][n] in original order;
 // qs is [n][kp] FP32 in sorted order, kp = K rounded up to 64, zero padded.

@@ -406,7 +406,6 @@ MacView mac_view(const btk_run_config &cfg, const Tree &tt, const Tree &ts) {
     v.w = m.w.p;
     v.log_budget = (float)(std::log10(m.budget) / kMacScale);
     v.di = n / 2 - 2;
-    v.P = ipow(n + 1, 3);
     v.T = MacPanels{tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p, tt.radius.p,
                     tt.count.p, tt.level.p};
     v.S = MacPanels{ts.lo[0].p, ts.lo[1].p, ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, ts.radius.p,
@@ -422,7 +421,6 @@ int mac_sample(const btk_run_config &cfg_in, int n, const double *x, const doubl
     auto t = build_tree(3, cfg.fmm_cluster_thresh, x, y, z, n, s);
     std::vector<int2> visited;
     auto l = build_list(&cfg, *t, *t, s, false, &visited);
-    // candidate pairs: disjoint boxes, separation ratio in [0.3, 2]
     std::vector<double> lo[3], hi[3], rad;
     std::vector<int> cnt(t->pc);
     for (int d = 0; d < 3; d++) {
@@ -434,8 +432,8 @@ int mac_sample(const btk_run_config &cfg_in, int n, const double *x, const doubl
     rad.resize(t->pc);
     BTK_HIP(hipMemcpy(rad.data(), t->radius.p, sizeof(double) * t->pc, hipMemcpyDeviceToHost));
     BTK_HIP(hipMemcpy(cnt.data(), t->count.p, sizeof(int) * t->pc, hipMemcpyDeviceToHost));
-    // the learned MAC's domain: a positive gap between the boxes, separation
-    // ratio in [0.3, kMacSepMax]
+    // candidate pairs: the learned MAC's domain, a positive gap between the
+    // boxes and separation ratio in [kMacSepMin, kMacSepMax)
     std::vector<int2> cand;
     for (const int2 &p : visited) {
         const int a = p.x, b = p.y;
@@ -449,7 +447,7 @@ int mac_sample(const btk_run_config &cfg_in, int n, const double *x, const doubl
         }
         if (!(g2 > 0) || !(e2 > 0)) continue;
         const double sep = (rad[a] + rad[b]) / std::sqrt(e2);
-        if (sep >= 0.3 && sep < kMacSepMax) cand.push_back(p);
+        if (sep >= kMacSepMin && sep < kMacSepMax) cand.push_back(p);
     }
     std::mt19937 rng(seed);
     std::shuffle(cand.begin(), cand.end(), rng);

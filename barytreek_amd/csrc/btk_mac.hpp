@@ -21,7 +21,9 @@ static_assert(kMacParams == BTK_MAC_WEIGHTS, "parameter count");
 
 // Output scale: network output = log10(relative error) / 4.
 constexpr float kMacScale = 4.0f;
-// Largest separation ratio (r_t + r_s)/d of a pair the learned MAC decides.
+// Separation ratios (r_t + r_s)/d of the pairs the learned MAC decides: the
+// range the training samples cover.
+constexpr float kMacSepMin = 0.3f;
 constexpr float kMacSepMax = 1.2f;
 
 // Panels as the MAC sees them.
@@ -65,9 +67,10 @@ __device__ __forceinline__ bool mac_features(int it, int is, const MacPanels &T,
     return true;
 }
 
-// Output o of the network (ReLU hidden layers) for features f; w may be in
-// global memory or LDS.
-__device__ __forceinline__ float mac_output(const float *w, const float *f, int o) {
+// Outputs o0 .. o0+NO-1 of the network (ReLU hidden layers) for features f;
+// w may be in global memory or LDS.
+template <int NO>
+__device__ __forceinline__ void mac_outputs(const float *w, const float *f, int o0, float *y) {
     float h1[kMacHid];  // fully unrolled loops keep it in registers
 #pragma unroll
     for (int i = 0; i < kMacHid; i++) {
@@ -76,42 +79,73 @@ __device__ __forceinline__ float mac_output(const float *w, const float *f, int 
         for (int j = 0; j < kMacIn; j++) a = fmaf(w[kMacW1 + i * kMacIn + j], f[j], a);
         h1[i] = fmaxf(a, 0.0f);
     }
-    float y = w[kMacB3 + o];
+#pragma unroll
+    for (int k = 0; k < NO; k++) y[k] = w[kMacB3 + o0 + k];
     for (int i = 0; i < kMacHid; i++) {
         float a = w[kMacB2 + i];
 #pragma unroll
         for (int j = 0; j < kMacHid; j++) a = fmaf(w[kMacW2 + i * kMacHid + j], h1[j], a);
-        y = fmaf(w[kMacW3 + o * kMacHid + i], fmaxf(a, 0.0f), y);
+        a = fmaxf(a, 0.0f);
+#pragma unroll
+        for (int k = 0; k < NO; k++) y[k] = fmaf(w[kMacW3 + (o0 + k) * kMacHid + i], a, y[k]);
     }
-    return y;
 }
 
 // What the traversal needs to consult the network.
 struct MacView {
     const float *w = nullptr;  // null: the theta MAC alone
     float log_budget = 0.0f;   // log10(budget) / 4, compared with the output
-    int di = 0;                // degree index (4, 6, 8, 10)
-    int P = 0;                 // proxy points per cluster at the degree in use
+    int di = 0;                // degree index of the configured degree (4, 6, 8, 10)
     MacPanels T{}, S{};
 };
 
-// The learned decision for a pair the theta MAC would refine: its type, or
-// -1 to leave the decision to the theta MAC.
-__device__ __forceinline__ int mac_accept(int it, int is, const MacView &m, const float *w, int thr) {
-    const int nt = m.T.count[it], ns = m.S.count[is];
-    const int type = (nt > thr ? 2 : 0) + (ns > thr ? 1 : 0);
-    if (type == 0) return -1;  // two small panels: only PP exists
-    const double direct = (double)nt * ns;
-    // the network is trusted only where it was trained: separated boxes (a
-    // positive gap, so no proxy point of one box meets one of the other)
-    // with separation ratio below kMacSepMax
-    const double approx = type == 1 ? (double)nt * m.P : (type == 2 ? (double)m.P * ns : (double)m.P * m.P);
-    if (!(approx < direct)) return -1;
+// Proxy points per cluster at degree index di (degrees 4, 6, 8, 10).
+__device__ __forceinline__ int mac_proxies(int di) {
+    const int n1 = 5 + 2 * di;
+    return n1 * n1 * n1;
+}
+
+// Predicted errors (network outputs) of an approximation of the given type
+// at every degree index up to m.di; false outside the trained domain:
+// separated boxes (a positive gap, so no proxy point of one box meets one of
+// the other) with separation ratio in [kMacSepMin, kMacSepMax).
+__device__ __forceinline__ bool mac_predict(int it, int is, const MacView &m, const float *w, int thr, int type,
+                                            float *y) {
     float f[kMacIn];
-    if (!mac_features(it, is, m.T, m.S, thr, f)) return -1;
-    if (!(f[8] > 0.0f) || f[0] >= kMacSepMax) return -1;
-    const float y = mac_output(w, f, (type - 1) * 4 + m.di);
-    return y <= m.log_budget ? type : -1;
+    if (!mac_features(it, is, m.T, m.S, thr, f)) return false;
+    if (!(f[8] > 0.0f) || f[0] < kMacSepMin || f[0] >= kMacSepMax) return false;
+    mac_outputs<4>(w, f, (type - 1) * 4, y);
+    return true;
+}
+
+// The learned decision for a pair of the given (non-PP) type, as a code:
+// type | degree index << 2.  A pair the theta MAC accepts gets the lowest
+// degree whose predicted error is within the budget (the configured degree
+// m.di outside the trained domain or when none is).  A pair the theta MAC
+// would refine is accepted at the lowest degree within the budget if that
+// approximation costs fewer evaluations than the direct sum; otherwise -1
+// leaves it to the theta MAC.
+__device__ __forceinline__ int mac_decide(int it, int is, const MacView &m, const float *w, int thr, int type,
+                                          bool accepted) {
+    const int nt = m.T.count[it], ns = m.S.count[is];
+    const double direct = (double)nt * ns;
+    auto approx = [&](int d) {
+        const double P = mac_proxies(d);
+        return type == 1 ? nt * P : (type == 2 ? P * ns : P * P);
+    };
+    if (accepted) {
+        if (m.di == 0) return type;
+    } else if (!(approx(0) < direct)) {
+        return -1;  // not even the lowest degree beats the direct sum
+    }
+    float y[4];
+    if (!mac_predict(it, is, m, w, thr, type, y)) return accepted ? type | (m.di << 2) : -1;
+    for (int d = 0; d <= m.di; d++) {
+        if (!(y[d] <= m.log_budget)) continue;
+        if (accepted) return type | (d << 2);
+        return approx(d) < direct ? (type | (d << 2)) : -1;
+    }
+    return accepted ? type | (m.di << 2) : -1;
 }
 
 // The installed model as the traversal of (tt, ts) uses it; throws when the

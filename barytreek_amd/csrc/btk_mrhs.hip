@@ -291,6 +291,7 @@ void fmm_interactions_multi(const btk_run_config &cfg, const btk_options &o, con
                             hipStream_t s, KernelEvents *kev) {
     BTK_REQUIRE(kp % KC == 0 && kp >= K, "charge-vector padding must be a multiple of 64");
     BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "many charge vectors: potential or potential+gradient kernels");
+    BTK_REQUIRE(l.ndeg == 1, "many charge vectors: one interpolation degree per list");
     if (tt.dim == 3) {
         if (o.kernel == BTK_KERNEL_POT)
             run_multi<3, BTK_KERNEL_POT>(cfg, o, tt, ts, l, qs, wt, K, kp, out, Psi, s, kev);

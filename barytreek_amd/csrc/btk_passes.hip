@@ -353,6 +353,72 @@ void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const Nod
     BTK_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// Change of interpolation degree, one workgroup per proxy array (a panel's
+// charges, or one component of a panel's potentials): out = (A x A x A) in
+// with A the nb x na 1-D matrix, applied as three 1-D contractions through
+// LDS; `add` accumulates into out.  Layout (j * n + k) * n + l as the passes.
+// ---------------------------------------------------------------------------
+struct DegreeMatrix {
+    double a[kMaxN1 * kMaxN1];  // row-major nb x na
+};
+
+__global__ void __launch_bounds__(TB)
+    k_degree_change(int na, int nb, DegreeMatrix A, const double *__restrict__ in, double *__restrict__ out,
+                    int add) {
+    __shared__ double X[kMaxP], Y[kMaxP];
+    const size_t b = blockIdx.x;
+    const int a2 = na * na, Pa = a2 * na, Pb = nb * nb * nb;
+    for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[b * Pa + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * a2; i += TB) {  // Y[j2][k1][l1] = sum_j1 A[j2][j1] X[j1][k1][l1]
+        const int j2 = i / a2, r = i % a2;
+        double v = 0.0;
+        for (int j1 = 0; j1 < na; j1++) v = fma(A.a[j2 * na + j1], X[j1 * a2 + r], v);
+        Y[i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * nb * na; i += TB) {  // X[j2][k2][l1] = sum_k1 A[k2][k1] Y[j2][k1][l1]
+        const int j2 = i / (nb * na), k2 = (i / na) % nb, l1 = i % na;
+        double v = 0.0;
+        for (int k1 = 0; k1 < na; k1++) v = fma(A.a[k2 * na + k1], Y[j2 * a2 + k1 * na + l1], v);
+        X[i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < Pb; i += TB) {  // out[j2][k2][l2] = sum_l1 A[l2][l1] X[j2][k2][l1]
+        const int jk = i / nb, l2 = i % nb;
+        double v = 0.0;
+        for (int l1 = 0; l1 < na; l1++) v = fma(A.a[l2 * na + l1], X[jk * na + l1], v);
+        if (add)
+            out[b * Pb + i] += v;
+        else
+            out[b * Pb + i] = v;
+    }
+}
+
+// A[i][k] = L_k(y_i): the degree-d Lagrange basis on the nodes cos(pi k / d)
+// at the degree-e proxy nodes cos(pi / e * i), in the reference interval.
+DegreeMatrix degree_matrix(int d, int e) {
+    const NodeTables td = node_tables(d), te = node_tables(e);
+    DegreeMatrix A{};
+    for (int i = 0; i <= e; i++)
+        for (int k = 0; k <= d; k++) {
+            double v = 1.0;
+            for (int j = 0; j <= d; j++)
+                if (j != k) v *= (te.b[i] - td.a[j]) / (td.a[k] - td.a[j]);
+            A.a[i * (d + 1) + k] = v;
+        }
+    return A;
+}
+
+// Row-major transpose of an (e+1) x (d+1) matrix.
+DegreeMatrix transposed(const DegreeMatrix &A, int d, int e) {
+    DegreeMatrix T{};
+    for (int i = 0; i <= e; i++)
+        for (int k = 0; k <= d; k++) T.a[k * (e + 1) + i] = A.a[i * (d + 1) + k];
+    return T;
+}
+
 }  // namespace
 
 void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_t s) {
@@ -406,6 +472,52 @@ void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &
         launch_l2p<3>(n1, lv, nl, t, tabs, ncomp, stride, Psi, out, s);
     else
         launch_l2p<2>(n1, lv, nl, t, tabs, ncomp, stride, Psi, out, s);
+}
+
+}  // namespace btk
+
+namespace btk {
+
+size_t proxy_offset(const List &l, int pc, int comps, int di) {
+    size_t off = 0;
+    for (int k = 0; k < di; k++) off += (size_t)comps * pc * ipow(l.deg[k] + 1, l.dim);
+    return off;
+}
+
+void upward_all(const btk_run_config &cfg, const List &l, const Tree &t, const double *q_sorted, double *W,
+                hipStream_t s) {
+    const int top = l.ndeg - 1, n = l.deg[top];
+    BTK_REQUIRE(n == cfg.interp_degree, "the list was built for another interp_degree");
+    double *Wn = W + proxy_offset(l, t.pc, 1, top);
+    upward_pass(cfg, t, q_sorted, Wn, s);
+    if (top == 0 || t.pc == 0) return;
+    BTK_REQUIRE(t.dim == 3, "several interpolation degrees: 3D only");
+    for (int di = 0; di < top; di++) {
+        // W_d[k] = sum_m W_n[m] L_k(y_m): A[k][m], degree d out, degree n in
+        const int d = l.deg[di];
+        const DegreeMatrix A = transposed(degree_matrix(d, n), d, n);
+        k_degree_change<<<t.pc, TB, 0, s>>>(n + 1, d + 1, A, Wn, W + proxy_offset(l, t.pc, 1, di), 0);
+        BTK_LAUNCH_CHECK();
+    }
+}
+
+void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l, const Tree &t, double *Psi,
+                  double *out, hipStream_t s) {
+    const int top = l.ndeg - 1, n = l.deg[top];
+    BTK_REQUIRE(n == cfg.interp_degree, "the list was built for another interp_degree");
+    const int C = components(t.dim, o.kernel);
+    double *Pn = Psi + proxy_offset(l, t.pc, C, top);
+    if (top > 0 && t.pc > 0) {
+        BTK_REQUIRE(t.dim == 3, "several interpolation degrees: 3D only");
+        for (int di = 0; di < top; di++) {
+            // Psi_n[m] += sum_k L_k(y_m) Psi_d[k]: degree n out, degree d in
+            const int d = l.deg[di];
+            k_degree_change<<<C * t.pc, TB, 0, s>>>(d + 1, n + 1, degree_matrix(d, n),
+                                                    Psi + proxy_offset(l, t.pc, C, di), Pn, 1);
+            BTK_LAUNCH_CHECK();
+        }
+    }
+    downward_pass(cfg, o, t, Pn, out, s);
 }
 
 }  // namespace btk

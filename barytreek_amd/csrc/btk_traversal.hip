@@ -30,7 +30,11 @@ TreeView view(const Tree &t) {
                     t.radius.p, t.count.p, t.child0.p};
 }
 
-// code: 0..3 emit with that type, 4 refine target, 5 refine source
+// Classification codes: below kRefineT an emitted pair, type | degree index
+// << 2 (the degree index is 0 but for the learned MAC); kRefineT refine the
+// target, kRefineS refine the source.
+constexpr int kRefineT = 16, kRefineS = 17;
+
 template <int DIM, bool MAC>
 __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, const TreeView &S, double theta,
                                              int thr, const MacView &mac, const float *mw) {
@@ -52,18 +56,23 @@ __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, 
     double sep = 100;
     if (dist > 0) sep = (T.radius[it] + S.radius[is]) / dist;
     const int ct = T.count[it], cs = S.count[is];
-    if (sep < theta) return (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);  // :155-179
+    const int type = (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);
+    const bool accepted = sep < theta;  // :155-179
     if constexpr (DIM == 3 && MAC) {
-        // the learned MAC may accept what the theta MAC would refine
-        const int c = mac_accept(it, is, mac, mw, thr);
-        if (c >= 0) return c;
+        // the learned MAC picks the degree of an accepted pair, and may accept
+        // what the theta MAC would refine
+        if (type != 0) {
+            const int c = mac_decide(it, is, mac, mw, thr, type, accepted);
+            if (c >= 0) return c;
+        }
     }
+    if (accepted) return type;
     // :180-203
     const bool lt = T.child0[it] < 0, lsf = S.child0[is] < 0;
     if (lt && lsf) return 0;
-    if (lt) return 5;
-    if (lsf) return 4;
-    return ct > cs ? 4 : 5;
+    if (lt) return kRefineS;
+    if (lsf) return kRefineT;
+    return ct > cs ? kRefineT : kRefineS;
 }
 
 template <int DIM, bool MAC>
@@ -74,7 +83,7 @@ __global__ void __launch_bounds__(TB) k_classify(int G, const int2 *__restrict__
     if (i >= G) return;
     const int c = classify_pair<DIM, MAC>(cur[i].x, cur[i].y, T, S, theta, thr, mac, mac.w);
     code[i] = (unsigned char)c;
-    val[i] = c < 4 ? 1ll : (1ll << 32);
+    val[i] = c < kRefineT ? 1ll : (1ll << 32);
 }
 
 // Emitted pair or refined children of pair i, given ex = emitted count |
@@ -84,7 +93,7 @@ __device__ __forceinline__ void write_pair(int c, int it, int is, long long ex, 
                                            const TreeView &S, int lbase, btk_interact_pair *__restrict__ out,
                                            int2 *__restrict__ next) {
     constexpr int NC = DIM == 3 ? 8 : 4;
-    if (c < 4) {
+    if (c < kRefineT) {
         btk_interact_pair p;
         p.target_panel = it;
         p.source_panel = is;
@@ -93,7 +102,7 @@ __device__ __forceinline__ void write_pair(int c, int it, int is, long long ex, 
     } else {
         const int k = (int)(ex >> 32);
         // :204-238: children pushed in c1..c8 order with the other side repeated
-        if (c == 4) {
+        if (c == kRefineT) {
             const int b = T.child0[it];
             for (int j = 0; j < NC; j++) next[NC * k + j] = make_int2(b + j, is);
         } else {
@@ -130,7 +139,7 @@ __global__ void __launch_bounds__(FB)
             it = cur[i].x;
             is = cur[i].y;
             c = classify_pair<DIM, MAC>(it, is, T, S, theta, thr, mac, mw);
-            v = c < 4 ? 1ll : (1ll << 32);
+            v = c < kRefineT ? 1ll : (1ll << 32);
         }
         long long incl = v;
 #pragma unroll
@@ -169,10 +178,12 @@ __global__ void __launch_bounds__(TB)
     write_pair<DIM>(code[i], cur[i].x, cur[i].y, inc[i] - val[i], T, S, lbase, out, next);
 }
 
-// Keys for the grouped forms: (far << tbits) | target.
-__global__ void k_group_keys(int L, const btk_interact_pair *__restrict__ pairs, int tbits, int pct,
+// Keys for the grouped forms: (group << tbits) | target, group = far * ndeg +
+// degree index.
+// Evaluations per type use the entry's degree: Pd[degree index] proxies.
+__global__ void k_group_keys(int L, const btk_interact_pair *__restrict__ pairs, int tbits, int pct, int ndeg,
                              int *__restrict__ keys, int *__restrict__ vals, int *__restrict__ cnt,
-                             const int *__restrict__ tcount, const int *__restrict__ scount, int P,
+                             const int *__restrict__ tcount, const int *__restrict__ scount, int4 Pd,
                              double *__restrict__ evals) {
     const int i = blockIdx.x * TB + threadIdx.x;
     __shared__ double se[4];
@@ -180,13 +191,15 @@ __global__ void k_group_keys(int L, const btk_interact_pair *__restrict__ pairs,
     __syncthreads();
     if (i < L) {
         const btk_interact_pair p = pairs[i];
-        const int far = p.interact_type >= 2 ? 1 : 0;
-        keys[i] = (far << tbits) | p.target_panel;
+        const int ty = p.interact_type & 3, di = p.interact_type >> 2;
+        const int P = di == 0 ? Pd.x : (di == 1 ? Pd.y : (di == 2 ? Pd.z : Pd.w));
+        const int grp = (ty >= 2 ? ndeg : 0) + di;
+        keys[i] = (grp << tbits) | p.target_panel;
         vals[i] = i;
-        atomicAdd(&cnt[far * pct + p.target_panel], 1);
-        const double nt = p.interact_type >= 2 ? (double)P : (double)tcount[p.target_panel];
-        const double ns = (p.interact_type & 1) ? (double)P : (double)scount[p.source_panel];
-        atomicAdd(&se[p.interact_type], nt * ns);
+        atomicAdd(&cnt[grp * pct + p.target_panel], 1);
+        const double nt = ty >= 2 ? (double)P : (double)tcount[p.target_panel];
+        const double ns = (ty & 1) ? (double)P : (double)scount[p.source_panel];
+        atomicAdd(&se[ty], nt * ns);
     }
     __syncthreads();
     if (threadIdx.x < 4 && se[threadIdx.x] != 0.0) atomicAdd(&evals[threadIdx.x], se[threadIdx.x]);
@@ -206,7 +219,7 @@ __global__ void k_type_count(int L, const btk_interact_pair *__restrict__ pairs,
     if (threadIdx.x < 4) sc[threadIdx.x] = 0;
     __syncthreads();
     const int i = blockIdx.x * TB + threadIdx.x;
-    if (i < L) atomicAdd(&sc[pairs[i].interact_type], 1u);
+    if (i < L) atomicAdd(&sc[pairs[i].interact_type & 3], 1u);
     __syncthreads();
     if (threadIdx.x < 4 && sc[threadIdx.x]) atomicAdd(&c[threadIdx.x], (unsigned long long)sc[threadIdx.x]);
 }
@@ -223,6 +236,14 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
     const TreeView T = view(tt), S = view(ts);
     long long *hv = host_scratch().v;
     const MacView mac = learned_mac ? mac_view(*cfg, tt, ts) : MacView{};
+    // the learned MAC picks each pair's degree among 4, 6, 8, 10 up to the
+    // configured one; otherwise every pair has the configured degree
+    if (learned_mac) {
+        l->ndeg = mac.di + 1;
+        for (int k = 0; k < l->ndeg; k++) l->deg[k] = 4 + 2 * k;
+    } else {
+        l->deg[0] = cfg->interp_degree;
+    }
 
     // generation 0: the root pair of every problem (one pair for one tree)
     BTK_REQUIRE(tt.nprob == ts.nprob, "target and source forests hold different numbers of problems");
@@ -324,20 +345,24 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
     const int pct = tt.pc;
     int tbits = 1;
     while ((1 << tbits) <= pct) tbits++;
-    const int P = ipow(cfg->interp_degree + 1, dim);
+    const int ng = 2 * l->ndeg;  // groups
+    int gbits = 1;
+    while ((1 << gbits) < ng) gbits++;
+    int Pd[4] = {0, 0, 0, 0};
+    for (int k = 0; k < l->ndeg; k++) Pd[k] = ipow(l->deg[k] + 1, dim);
     DBuf<int> keys(L ? L : 1), vals(L ? L : 1), keys2(L ? L : 1), order(L ? L : 1);
-    DBuf<int> cnt(2 * pct + 1);
+    DBuf<int> cnt((size_t)ng * pct + 1);
     DBuf<double> ev(4);
-    BTK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * pct + 1), s));
+    BTK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)ng * pct + 1), s));
     BTK_HIP(hipMemsetAsync(ev.p, 0, sizeof(double) * 4, s));
     if (L) {
-        k_group_keys<<<nblk(L), TB, 0, s>>>(L, l->pairs.p, tbits, pct, keys.p, vals.p, cnt.p, tt.count.p,
-                                            ts.count.p, P, ev.p);
+        k_group_keys<<<nblk(L), TB, 0, s>>>(L, l->pairs.p, tbits, pct, l->ndeg, keys.p, vals.p, cnt.p, tt.count.p,
+                                            ts.count.p, make_int4(Pd[0], Pd[1], Pd[2], Pd[3]), ev.p);
         BTK_LAUNCH_CHECK();
-        sort_pairs_i32(keys.p, keys2.p, vals.p, order.p, L, tbits + 1, s);
+        sort_pairs_i32(keys.p, keys2.p, vals.p, order.p, L, tbits + gbits, s);
     }
-    l->goff.resize(2 * pct + 1);
-    exclusive_sum_i32(cnt.p, l->goff.p, 2 * pct + 1, s);
+    l->goff.resize((size_t)ng * pct + 1);
+    exclusive_sum_i32(cnt.p, l->goff.p, ng * pct + 1, s);
     l->gsrc.resize(L ? L : 1);
     l->gtype.resize(L ? L : 1);
     if (L) {

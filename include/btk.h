@@ -174,10 +174,22 @@ int btk_dual_tree_traversal_2d(btk_run_config *cfg, const btk_tree *target, cons
                                btk_list **list);
 int btk_dual_tree_traversal_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
                                btk_list **list);
+/* The 3D traversal with the installed learned MAC (btk_mac_train /
+ * btk_mac_set_weights; no counterpart in the reference): the theta MAC's
+ * pairs plus those the network accepts within the budget
+ * (btk_mac_set_budget), each approximated pair at the lowest of the degrees
+ * 4, 6, 8, 10 (up to cfg->interp_degree, which must be one of them) whose
+ * predicted error is within the budget.  interact_type holds the type in
+ * bits 0-1 and the degree index (degree 4 + 2 * index) in bits 2-3.  Such a
+ * list is for inspection and btk_split_interactions; btk_solve with
+ * btk_options.mac runs it.  Counters as btk_dual_tree_traversal_3d. */
+int btk_dual_tree_traversal_learned_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source,
+                                       void *stream, btk_list **list);
 int btk_list_count(const btk_list *list);
 /* The full list in the reference's FIFO order (host array of count). */
 int btk_list_copy(const btk_list *list, btk_interact_pair *pairs);
-/* split_interactions (interaction_list.hpp:10): stable split by type. */
+/* split_interactions (interaction_list.hpp:10): stable split by type
+ * (interact_type & 3). */
 int btk_split_interactions(const btk_list *list, btk_interact_pair *pp, btk_interact_pair *pc,
                            btk_interact_pair *cp, btk_interact_pair *cc);
 void btk_list_destroy(btk_list *list);
@@ -255,13 +267,20 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
  * both levels, log2(r_t/r_s), g/d and (r_t + r_s)/g with g the distance
  * between the boxes) and predicts log10 of the relative error of its PC, CP
  * and CC approximations at degrees 4, 6, 8 and 10 (output 4*type' + degree
- * index, type' = PC 0, CP 1, CC 2; scaled by 1/4, clamped to [-14, 0]).  With opts->mac = 1 (3D only, degree 4, 6,
- * 8 or 10) the traversal evaluates it on every pair that the theta MAC
- * would refine whose boxes are apart (a positive gap) with separation ratio
- * below 1.2, and emits the pair at its count-given type when the predicted
- * error is within the budget and the approximation costs fewer evaluations
- * than the direct sum of the pair; every other pair follows the theta MAC,
- * which stays the fallback.  The model and budget are process-wide. */
+ * index, type' = PC 0, CP 1, CC 2; scaled by 1/4, clamped to [-14, 0]).
+ * With opts->mac = 1 (3D only; interp_degree 4, 6, 8 or 10, the highest
+ * degree used) the traversal evaluates it on the non-PP pairs whose boxes are
+ * apart (a positive gap) with separation ratio in [0.3, 1.2), the network's
+ * domain.  A pair the theta MAC accepts is approximated at the lowest degree
+ * whose predicted error is within the budget (interp_degree when none is or
+ * outside the domain).  A pair the theta MAC would refine is emitted at its
+ * count-given type and the lowest degree within the budget when that costs
+ * fewer evaluations than the direct sum of the pair; otherwise the theta MAC
+ * refines it, as it decides every pair outside the domain.  The solve runs
+ * the upward pass at interp_degree, restricts the charges to each lower
+ * degree, and extends each lower degree's potentials into interp_degree's
+ * before one downward pass (exact polynomial identities, to rounding).  The
+ * model and budget are process-wide. */
 #define BTK_MAC_FEATURES 10
 #define BTK_MAC_OUTPUTS 12
 #define BTK_MAC_WEIGHTS 5644

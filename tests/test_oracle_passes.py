@@ -13,7 +13,9 @@ without the reference binary:
 - downward (downward_pass_impl.hpp): a field of degree <= n given on the root's
   target proxies reaches every point exactly (parent->child and leaf->point
   interpolate polynomials exactly);
-- whole path: the error against the direct sum falls as the degree rises.
+- whole path: the error against the direct sum falls as the degree rises;
+- degree change: the lower-degree upward and downward passes from the
+  higher-degree ones through the 1-D basis matrix between the two degrees.
 """
 import numpy as np
 import pytest
@@ -201,3 +203,45 @@ def test_error_falls_with_degree():
         errs.append(np.linalg.norm(out - ds) / np.linalg.norm(ds))
     assert errs[0] > errs[1] > errs[2]
     assert errs[2] < 1e-5
+
+
+def _degree_matrix(d, e):
+    """A[i][k] = L_k(y_i): the degree-d Lagrange basis on the nodes
+    cos(pi k / d) at the degree-e proxy nodes cos(pi / e * i) of [-1, 1]."""
+    a, b = np.cos(PI * np.arange(d + 1) / d), np.cos(PI / e * np.arange(e + 1))
+    A = np.ones((e + 1, d + 1))
+    for k in range(d + 1):
+        for j in range(d + 1):
+            if j != k:
+                A[:, k] *= (b - a[j]) / (a[k] - a[j])
+    return A
+
+
+@pytest.mark.parametrize("d", [4, 6])
+def test_degree_change_identities(d):
+    """The two identities the solve with per-pair degrees stands on, on the
+    oracle's own passes: the degree-d upward pass equals the degree-n one
+    restricted by A^T in every dimension, and the degree-d downward pass of
+    proxy potentials equals the degree-n one of the potentials extended by A
+    (degree-d polynomials are reproduced by degree-n interpolation)."""
+    n, thresh = 8, 60
+    x, y, z, _ = wl.plummer_3d(2000, 4, a=0.3, rmax=3.0)
+    q = np.random.default_rng(2).uniform(-1, 1, 2000)
+    t = O.OracleTree(3, thresh, x, y, z)
+    cn, cd = O.make_config(3, n, thresh, 0.7, 0.0), O.make_config(3, d, thresh, 0.7, 0.0)
+    A = _degree_matrix(d, n)
+    Wn = O.upward_pass(cn, t, q).reshape(-1, n + 1, n + 1, n + 1)
+    Wd = O.upward_pass(cd, t, q)
+    got = np.einsum("ak,bl,cm,pabc->pklm", A, A, A, Wn).reshape(Wd.shape)
+    assert np.abs(got - Wd).max() <= 1e-13 * np.abs(Wd).max()
+    o = O.opts(kernel=O.KERNEL_POT)
+    Psi_d = np.random.default_rng(3).normal(size=(1, t.panel_count, (d + 1) ** 3))
+    ref = O.downward_pass(cd, o, t, Psi_d, np.zeros((1, 2000)))
+    Psi_n = np.einsum("ak,bl,cm,pklm->pabc", A, A, A, Psi_d[0].reshape(-1, d + 1, d + 1, d + 1))
+    got = O.downward_pass(cn, o, t, Psi_n.reshape(1, t.panel_count, -1), np.zeros((1, 2000)))
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the proxy nodes of degree n in place of its basis nodes' own degree-d
+    # counterparts break the restriction: the identity is not vacuous
+    A_bad = _degree_matrix(d, n)[::-1]
+    bad = np.einsum("ak,bl,cm,pabc->pklm", A_bad, A, A, Wn).reshape(Wd.shape)
+    assert np.abs(bad - Wd).max() > 1e-6 * np.abs(Wd).max()
