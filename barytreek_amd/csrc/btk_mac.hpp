@@ -105,47 +105,63 @@ __device__ __forceinline__ int mac_proxies(int di) {
     return n1 * n1 * n1;
 }
 
-// Predicted errors (network outputs) of an approximation of the given type
-// at every degree index up to m.di; false outside the trained domain:
-// separated boxes (a positive gap, so no proxy point of one box meets one of
-// the other) with separation ratio in [kMacSepMin, kMacSepMax).
-__device__ __forceinline__ bool mac_predict(int it, int is, const MacView &m, const float *w, int thr, int type,
-                                            float *y) {
+// Predicted errors (network outputs) of the pair's PC, CP and CC
+// approximations at every degree (output 4 * (type - 1) + degree index);
+// false outside the trained domain: separated boxes (a positive gap, so no
+// proxy point of one box meets one of the other) with separation ratio in
+// [kMacSepMin, kMacSepMax).
+__device__ __forceinline__ bool mac_predict(int it, int is, const MacView &m, const float *w, float *y) {
     float f[kMacIn];
-    if (!mac_features(it, is, m.T, m.S, thr, f)) return false;
+    if (!mac_features(it, is, m.T, m.S, 0, f)) return false;
     if (!(f[8] > 0.0f) || f[0] < kMacSepMin || f[0] >= kMacSepMax) return false;
-    mac_outputs<4>(w, f, (type - 1) * 4, y);
+    mac_outputs<kMacOut>(w, f, 0, y);
     return true;
 }
 
-// The learned decision for a pair of the given (non-PP) type, as a code:
-// type | degree index << 2.  A pair the theta MAC accepts gets the lowest
-// degree whose predicted error is within the budget (the configured degree
-// m.di outside the trained domain or when none is).  A pair the theta MAC
-// would refine is accepted at the lowest degree within the budget if that
-// approximation costs fewer evaluations than the direct sum; otherwise -1
-// leaves it to the theta MAC.
+// The learned decision for a pair, as a code type | degree index << 2, or -1
+// to leave the pair to the theta MAC.  type is the theta MAC's count-given
+// type.  The candidates are the theta MAC's own choice when it accepts (type
+// at the configured degree m.di, or PP), and every approximation at a degree
+// up to m.di whose predicted error is within the budget: CC for any two
+// panels, PC and CP where the panel on the point side holds at most thr
+// points (a leaf, as in the reference's types); the pair takes the candidate
+// with the fewest kernel evaluations.  A pair the theta MAC would refine is
+// accepted only by an approximation cheaper than its direct sum.
 __device__ __forceinline__ int mac_decide(int it, int is, const MacView &m, const float *w, int thr, int type,
                                           bool accepted) {
-    const int nt = m.T.count[it], ns = m.S.count[is];
-    const double direct = (double)nt * ns;
-    auto approx = [&](int d) {
+    const int ct = m.T.count[it], cs = m.S.count[is];
+    const double nt = ct, ns = cs;
+    const double direct = nt * ns;
+    auto cost = [&](int t, int d) {
         const double P = mac_proxies(d);
-        return type == 1 ? nt * P : (type == 2 ? P * ns : P * P);
+        return t == 1 ? nt * P : (t == 2 ? P * ns : P * P);
     };
+    double best = direct;
+    int code = -1;
     if (accepted) {
-        if (m.di == 0) return type;
-    } else if (!(approx(0) < direct)) {
-        return -1;  // not even the lowest degree beats the direct sum
+        code = type == 0 ? 0 : (type | (m.di << 2));
+        best = type == 0 ? direct : cost(type, m.di);
     }
-    float y[4];
-    if (!mac_predict(it, is, m, w, thr, type, y)) return accepted ? type | (m.di << 2) : -1;
-    for (int d = 0; d <= m.di; d++) {
-        if (!(y[d] <= m.log_budget)) continue;
-        if (accepted) return type | (d << 2);
-        return approx(d) < direct ? (type | (d << 2)) : -1;
+    const bool pc_ok = ct <= thr, cp_ok = cs <= thr;
+    // the cheapest approximation must beat the best so far
+    double low = cost(3, 0);
+    if (pc_ok) low = fmin(low, cost(1, 0));
+    if (cp_ok) low = fmin(low, cost(2, 0));
+    if (!(low < best)) return code;
+    float y[kMacOut];
+    if (!mac_predict(it, is, m, w, y)) return code;
+#pragma unroll
+    for (int t = 1; t <= 3; t++) {
+        if ((t == 1 && !pc_ok) || (t == 2 && !cp_ok)) continue;
+        for (int d = 0; d <= m.di; d++) {
+            const double c = cost(t, d);
+            if (y[(t - 1) * 4 + d] <= m.log_budget && c < best) {
+                best = c;
+                code = t | (d << 2);
+            }
+        }
     }
-    return accepted ? type | (m.di << 2) : -1;
+    return code;
 }
 
 // The installed model as the traversal of (tt, ts) uses it; throws when the

@@ -59,12 +59,10 @@ __device__ __forceinline__ int classify_pair(int it, int is, const TreeView &T, 
     const int type = (ct > thr ? 2 : 0) + (cs > thr ? 1 : 0);
     const bool accepted = sep < theta;  // :155-179
     if constexpr (DIM == 3 && MAC) {
-        // the learned MAC picks the degree of an accepted pair, and may accept
-        // what the theta MAC would refine
-        if (type != 0) {
-            const int c = mac_decide(it, is, mac, mw, thr, type, accepted);
-            if (c >= 0) return c;
-        }
+        // the learned MAC picks the type and degree of an accepted pair, and
+        // may accept what the theta MAC would refine
+        const int c = mac_decide(it, is, mac, mw, thr, type, accepted);
+        if (c >= 0) return c;
     }
     if (accepted) return type;
     // :180-203
@@ -115,10 +113,16 @@ __device__ __forceinline__ void write_pair(int c, int it, int is, long long ex, 
 // A small generation in one workgroup: classify, exclusive scan of the
 // packed counts, writes; the totals (emitted | refined << 32) go to *tot in
 // pinned host memory.
-constexpr int FB = 1024;
 constexpr int kFuseMax = 8192;
 
-template <int DIM, bool MAC>
+// Threads of the fused generation: 1024, or 512 with the learned MAC, whose
+// network needs the registers of two wavefronts per SIMD.
+template <bool MAC>
+constexpr int fused_threads() {
+    return MAC ? 512 : 1024;
+}
+
+template <int DIM, bool MAC, int FB = fused_threads<MAC>()>
 __global__ void __launch_bounds__(FB)
     k_generation(int G, const int2 *__restrict__ cur, TreeView T, TreeView S, double theta, int thr, MacView mac,
                  int lbase, btk_interact_pair *__restrict__ out, int2 *__restrict__ next, long long *tot) {
@@ -275,7 +279,7 @@ std::unique_ptr<List> build_list(btk_run_config *cfg, const Tree &tt, const Tree
             reserve((size_t)L + G);
             nxt.resize((size_t)NC * G);
 #define BTK_GEN(D, M)                                                                                      \
-    k_generation<D, M><<<1, FB, 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, mac, L, l->pairs.p, \
+    k_generation<D, M><<<1, fused_threads<M>(), 0, s>>>(G, cur.p, T, S, cfg->fmm_theta, cfg->fmm_cluster_thresh, mac, L, l->pairs.p, \
                                         nxt.p, hv)
             if (dim == 3 && mac.w)
                 BTK_GEN(3, true);

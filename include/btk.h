@@ -175,12 +175,11 @@ int btk_dual_tree_traversal_2d(btk_run_config *cfg, const btk_tree *target, cons
 int btk_dual_tree_traversal_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source, void *stream,
                                btk_list **list);
 /* The 3D traversal with the installed learned MAC (btk_mac_train /
- * btk_mac_set_weights; no counterpart in the reference): the theta MAC's
- * pairs plus those the network accepts within the budget
- * (btk_mac_set_budget), each approximated pair at the lowest of the degrees
- * 4, 6, 8, 10 (up to cfg->interp_degree, which must be one of them) whose
- * predicted error is within the budget.  interact_type holds the type in
- * bits 0-1 and the degree index (degree 4 + 2 * index) in bits 2-3.  Such a
+ * btk_mac_set_weights and btk_mac_set_budget; no counterpart in the
+ * reference; the decision rule is described with btk_mac_sample below):
+ * each pair's type and degree among 4, 6, 8, 10 (up to cfg->interp_degree,
+ * which must be one of them).  interact_type holds the type in bits 0-1 and
+ * the degree index (degree 4 + 2 * index) in bits 2-3.  Such a
  * list is for inspection and btk_split_interactions; btk_solve with
  * btk_options.mac runs it.  Counters as btk_dual_tree_traversal_3d. */
 int btk_dual_tree_traversal_learned_3d(btk_run_config *cfg, const btk_tree *target, const btk_tree *source,
@@ -271,12 +270,15 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
  * With opts->mac = 1 (3D only; interp_degree 4, 6, 8 or 10, the highest
  * degree used) the traversal evaluates it on the non-PP pairs whose boxes are
  * apart (a positive gap) with separation ratio in [0.3, 1.2), the network's
- * domain.  A pair the theta MAC accepts is approximated at the lowest degree
- * whose predicted error is within the budget (interp_degree when none is or
- * outside the domain).  A pair the theta MAC would refine is emitted at its
- * count-given type and the lowest degree within the budget when that costs
- * fewer evaluations than the direct sum of the pair; otherwise the theta MAC
- * refines it, as it decides every pair outside the domain.  The solve runs
+ * domain.  Its candidates for a pair are the approximations at degrees up to
+ * interp_degree whose predicted error is within the budget: CC for any two
+ * panels, PC and CP where the panel on the point side is a leaf (at most
+ * fmm_cluster_thresh points, as in the reference's types).  A pair the theta
+ * MAC accepts takes the cheapest (fewest kernel evaluations) of these and the
+ * theta MAC's own choice (its count-given type at interp_degree).  A pair the
+ * theta MAC would refine takes the cheapest candidate that costs fewer
+ * evaluations than the pair's direct sum; otherwise the theta MAC refines
+ * it, as it decides every pair outside the domain.  The solve runs
  * the upward pass at interp_degree, restricts the charges to each lower
  * degree, and extends each lower degree's potentials into interp_degree's
  * before one downward pass (exact polynomial identities, to rounding).  The

@@ -83,6 +83,13 @@ def test_learned_degrees_match_oracle_degree_by_degree(kernel):
     hist = [int(np.sum((ty != 0) & (di == k))) for k in range(3)]
     print(f"  approximated pairs at degree 4/6/8: {hist}", flush=True)
     assert min(hist) > 0
+    # the point side of PC and CP is a leaf (at most THR points), as in the
+    # reference's types; CC may join two leaves
+    cnt = tree.panels()["point_count"]
+    tc, sc = cnt[pairs["target_panel"]], cnt[pairs["source_panel"]]
+    assert np.all(tc[ty == 1] <= THR) and np.all(sc[ty == 2] <= THR)
+    assert np.all((tc[ty == 0] <= THR) & (sc[ty == 0] <= THR))
+    print(f"  CC pairs of two leaves: {int(np.sum((ty == 3) & (tc <= THR) & (sc <= THR)))}", flush=True)
     C = btk.components(3, kernel)
     out = torch.zeros(C * N, dtype=torch.float64, device="cuda")
     st = btk.solve(cfg, btk.options(kernel=kernel, exclude_self=1, mac=1), d[0], d[1], d[2], d[3], out)
