@@ -18,6 +18,7 @@ sampled targets above).  Launch shapes are autotuned before the warmup.
                                 3d-batched-2M|3d-forest-256]
                     [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
                     [--no-cpu-baseline | --cpu-baseline] [--no-accuracy] [--no-degree-scan]
+                    [--mac-budget B]
 """
 from __future__ import annotations
 
@@ -85,6 +86,8 @@ def parse():
                     help="time the CPU oracle also on workloads above 1M points (minutes of CPU time)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--mac-budget", type=float, default=None,
+                    help="learned MAC: this per-pair budget instead of the scan (for profiling)")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
@@ -94,7 +97,7 @@ def parse():
 
 
 # per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (1e-5, 6e-6, 4e-6, 3e-6, 2e-6, 1e-6, 3e-7)
+MAC_BUDGETS = (1e-5, 8e-6, 6e-6, 5e-6, 4e-6, 3.5e-6, 3e-6, 2e-6, 1e-6, 3e-7)
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta):
@@ -462,7 +465,7 @@ def main():
         if v == "learned":
             # the largest per-pair budget whose result stays within 1e-6
             scan = []
-            for b in MAC_BUDGETS:
+            for b in ([args.mac_budget] if args.mac_budget else MAC_BUDGETS):
                 btk.mac_set_budget(b)
                 log(f"timing learned, budget {b:g}")
                 ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
