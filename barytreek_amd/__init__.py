@@ -68,7 +68,7 @@ class Options(C.Structure):
 class Tuning(C.Structure):
     """btk_tuning: launch shapes of the interaction kernels (0 = default)."""
     _fields_ = [("near_items", C.c_int), ("far_items", C.c_int), ("near_tpt", C.c_int), ("far_tpt", C.c_int),
-                ("mutual", C.c_int), ("reserved", C.c_int * 3)]
+                ("mutual", C.c_int), ("cc_group", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class SolveStats(C.Structure):
@@ -76,7 +76,8 @@ class SolveStats(C.Structure):
                 ("t_interact", C.c_double), ("t_downward", C.c_double), ("t_total", C.c_double),
                 ("count", C.c_longlong * 4), ("evals", C.c_double * 4), ("t_kernel_near", C.c_double),
                 ("t_kernel_far", C.c_double), ("panel_count_target", C.c_int),
-                ("panel_count_source", C.c_int), ("levels_target", C.c_int), ("levels_source", C.c_int)]
+                ("panel_count_source", C.c_int), ("levels_target", C.c_int), ("levels_source", C.c_int),
+                ("cc_shared", C.c_longlong)]
 
 
 PAIR_DTYPE = np.dtype([("target_panel", "<i4"), ("source_panel", "<i4"), ("interact_type", "<i4")])
@@ -180,11 +181,12 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0) -> None:
-    """Process-wide launch shapes of the interaction kernels; 0 = default."""
+def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0, cc_group=0) -> None:
+    """Process-wide launch shapes of the interaction kernels; 0 = default
+    (cc_group: -1 turns the shared-matrix CC products off)."""
     t = Tuning()
     t.near_items, t.far_items, t.near_tpt, t.far_tpt = near_items, far_items, near_tpt, far_tpt
-    t.mutual = mutual
+    t.mutual, t.cc_group = mutual, cc_group
     _check(lib().btk_set_tuning(C.byref(t)))
 
 

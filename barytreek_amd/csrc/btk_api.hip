@@ -140,6 +140,7 @@ int btk_set_tuning(const btk_tuning *t) {
         BTK_REQUIRE(t->near_tpt >= 0 && t->near_tpt <= 2 && t->far_tpt >= 0 && t->far_tpt <= 2,
                     "targets per lane must be 0 (default), 1 or 2");
         BTK_REQUIRE(t->mutual == 0 || t->mutual == 1, "mutual must be 0 or 1");
+        BTK_REQUIRE(t->cc_group >= -1, "cc_group must be -1 (off), 0 (default) or a minimum group size");
         btk::tuning() = *t;
     });
 }
@@ -545,6 +546,7 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
             }
             stats->t_kernel_near = 1e-3 * ms_near;
             stats->t_kernel_far = 1e-3 * ms_far;
+            stats->cc_shared = kev.cc_shared;
             stats->panel_count_target = tt.pc;
             stats->panel_count_source = ts->pc;
             stats->levels_target = tt.levels;
@@ -815,6 +817,7 @@ int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int n
             }
             stats->t_kernel_near = 1e-3 * ms_near;
             stats->t_kernel_far = 1e-3 * ms_far;
+            stats->cc_shared = kev.cc_shared;
             stats->panel_count_target = stats->panel_count_source = t->pc;
             stats->levels_target = stats->levels_source = t->levels;
         }
@@ -894,6 +897,7 @@ int btk_solve_multi(const btk_run_config *cfg_in, const btk_options *opts, int m
             }
             stats->t_kernel_near = 1e-3 * ms_near;
             stats->t_kernel_far = 1e-3 * ms_far;
+            stats->cc_shared = kev.cc_shared;
             stats->panel_count_target = tt.pc;
             stats->panel_count_source = ts->pc;
             stats->levels_target = tt.levels;

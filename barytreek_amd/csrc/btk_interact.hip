@@ -139,6 +139,7 @@ template <int DIM, int KERNEL>
 void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
                 hipStream_t s, KernelEvents *kev) {
     const Tree &tt = *p.tt, &ts = *p.ts;
+    if (kev) kev->cc_shared = p.cc_shared;
     for (int far = 0; far < 2; far++) {
         bool any = false;
         for (int di = 0; di < p.ndeg; di++) any |= p.g[far][di].present;
@@ -171,7 +172,9 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.out = dst;
             L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
             if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, s);
-            launch_interact<DIM, KERNEL>(L, s);
+            if (g.cc.n_items)
+                launch_ccgemm(g.cc, tt, ts, W + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride, s);
+            if (g.tk.n) launch_interact<DIM, KERNEL>(L, s);
         }
         if (kev) {
             BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
@@ -324,7 +327,14 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             // for both directions (PP entries are in the degree-0 near group)
             g.mutual = !far && di == 0 && f32 && &tt == &ts && tu.mutual;
             if (g.mutual) build_mutual(l, tt, g.tpt, p->meta.p, g.mu, s);
-            const int4 *meta = g.mutual ? p->meta.p : nullptr;
+            // mixed precision, 3D: CC pairs that share a kernel matrix run as
+            // MFMA products
+            if (far && f32 && dim == 3 && o.kernel != BTK_KERNEL_REF_VEL3D && tu.cc_group >= 0) {
+                build_ccgemm(l, tt, ts, grp, l.deg[di] + 1, tu.cc_group > 0 ? tu.cc_group : kCcMinGroup,
+                             p->meta.p, g.cc, s);
+                p->cc_shared += g.cc.shared;
+            }
+            const int4 *meta = (g.mutual || g.cc.n_items > 0) ? p->meta.p : nullptr;
             long long base;
             if (l.ndeg == 1) {  // one degree: work items of the whole class, bounded from the tree
                 const int n_clusters = tt.pc - tt.n_leaves;
@@ -337,7 +347,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
                                         : (tu.near_items ? tu.near_items : kNearItems);
             g.nsplit = choose_split(base, items);
             build_tasks(l, tt, ts, grp, P, per, g.nsplit, g.tk, s, meta);
-            g.present = g.tk.n > 0 || g.mutual;
+            g.present = g.tk.n > 0 || g.mutual || g.cc.n_items > 0;
         }
     }
     return p;

@@ -19,9 +19,11 @@
 // written once.
 //
 // Precision.  FP64: t - s and the kernel in FP64 (the reference's arithmetic).
-// Mixed (F32): t - s formed in FP64 and rounded once to FP32, the kernel in
-// FP32 (hardware log2 / reciprocal / reciprocal square root), FP32 partial
-// sums over one 64-source tile, added into FP64 per-target accumulators.
+// Mixed (F32): t - s to FP32 relative precision (near 3D: FP32 hi/lo pairs,
+// TileSplit; near 2D: formed in FP64 and rounded once; far: relative to the
+// target box centre, TileRel), the kernel in FP32 (hardware log2 / reciprocal
+// / reciprocal square root), FP32 partial sums over one 64-source tile, added
+// into FP64 per-target accumulators.
 #pragma once
 
 #include <type_traits>
@@ -71,6 +73,24 @@ struct TileSrc {
 struct TileRel {
     float x, y, z, q;
 };
+
+// A source of the 3D near (PP, PC) kernels in mixed precision: each
+// coordinate as an FP32 pair, hi = fl32(x) and lo = fl32(x - hi).  t - s is
+// formed as (t_hi - s_hi) + (t_lo - s_lo): the first difference is exact when
+// the two hi values lie within a factor two of each other (close points) and
+// otherwise rounds relative to t - s itself, the second carries the residuals,
+// so t - s keeps FP32 relative precision down to separations of about
+// 2^-48 |t| -- as forming it in FP64 and rounding does, in three FP32
+// operations instead of an FP64 subtraction and a conversion.  Points closer
+// than that count as coincident under exclude_self.
+struct TileSplit {
+    float xh, yh, zh, q, xl, yl, zl, pad;
+};
+
+__device__ __forceinline__ void split_f32(double x, float &hi, float &lo) {
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+}
 
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
 // sorted point of its points (0 for proxy sources), z = number of sources
@@ -145,6 +165,27 @@ __device__ __forceinline__ TileRel load_source_rel(const int4 &m, int j, int n1,
     return TileRel{(float)(a.x - c[0]), (float)(a.y - c[1]), DIM == 3 ? (float)(a.z - c[2]) : 0.0f, a.q};
 }
 
+template <int DIM>
+__device__ __forceinline__ TileSplit load_source_split(const int4 &m, int j, int n1, const NodeTables &tabs,
+                                                       const SrcView &sv) {
+    const TileSrc<true> a = load_source<DIM, true>(m, j, n1, tabs, sv);
+    TileSplit t{};
+    split_f32(a.x, t.xh, t.xl);
+    split_f32(a.y, t.yh, t.yl);
+    split_f32(a.z, t.zh, t.zl);
+    t.q = a.q;
+    return t;
+}
+
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_pair_split(const TileSplit &s, const float *th, const float *tl, float eps2,
+                                                float *acc32) {
+    const float dx = (th[0] - s.xh) + (tl[0] - s.xl);
+    const float dy = (th[1] - s.yh) + (tl[1] - s.yl);
+    const float dz = DIM == 3 ? (th[2] - s.zh) + (tl[2] - s.zl) : 0.0f;
+    eval_f32<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc32);
+}
+
 template <int DIM, int KERNEL, bool EXCL>
 __device__ __forceinline__ void eval_pair_rel(const TileRel &s, const float *t, float eps2, float *acc32) {
     eval_f32<DIM, KERNEL, EXCL>(t[0] - s.x, t[1] - s.y, DIM == 3 ? t[2] - s.z : 0.0f, s.q, eps2, acc32);
@@ -170,8 +211,9 @@ __global__ void __launch_bounds__(kLanes)
     k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
-    constexpr bool REL = F32 && FAR;  // relative FP32 coordinates (TileRel)
-    using Tile = std::conditional_t<REL, TileRel, TileSrc<F32>>;
+    constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
+    constexpr bool SPLIT = F32 && !FAR && DIM == 3;  // FP32 hi/lo coordinates (TileSplit)
+    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<SPLIT, TileSplit, TileSrc<F32>>>;
     __shared__ Tile tile[kLanes];
     const int lane = threadIdx.x;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -182,7 +224,7 @@ __global__ void __launch_bounds__(kLanes)
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
     double t[TPT][3];
-    float tr[TPT][3];
+    float tr[TPT][3], th[TPT][3], tl[TPT][3];
     double acc[TPT][C];
     double ctr[3] = {0.0, 0.0, 0.0};
     if constexpr (REL) {
@@ -203,13 +245,18 @@ __global__ void __launch_bounds__(kLanes)
             t[k][2] = DIM == 3 ? tv.sz[ts0 + ti] : 0.0;
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) tr[k][d] = (float)(t[k][d] - ctr[d]);
+        for (int d = 0; d < 3; d++) {
+            tr[k][d] = (float)(t[k][d] - ctr[d]);
+            split_f32(t[k][d], th[k][d], tl[k][d]);
+        }
 #pragma unroll
         for (int c = 0; c < C; c++) acc[k][c] = 0.0;
     }
     auto fetch = [&](const int4 &mm, int j) -> Tile {
         if constexpr (REL)
             return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
+        else if constexpr (SPLIT)
+            return load_source_split<DIM>(mm, j, n1, tabs, sv);
         else
             return load_source<DIM, F32>(mm, j, n1, tabs, sv);
     };
@@ -217,6 +264,8 @@ __global__ void __launch_bounds__(kLanes)
         constexpr bool X = decltype(excl)::value;
         if constexpr (REL)
             eval_pair_rel<DIM, KERNEL, X>(s, tr[k], (float)kp.eps2, a32);
+        else if constexpr (SPLIT)
+            eval_pair_split<DIM, KERNEL, X>(s, th[k], tl[k], (float)kp.eps2, a32);
         else
             eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
     };
@@ -344,6 +393,29 @@ extern template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, cons
 extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
                                                            const KParams &, double *, hipStream_t);
 
+// CC pairs sharing a kernel matrix (btk_mrhs.hip), mixed precision, 3D.
+// Panel boxes are exact halves of their parents, so the P x P kernel matrix of
+// a CC pair depends only on the two levels and the offset of the box centres
+// (a multiple of half the finer box per axis).  The CC entries of one far
+// group are keyed by (target level, source level, offset) and every key held
+// by at least min_group entries becomes one MFMA product: the matrix (from
+// the key's first pair) times the pairs' proxy weights as columns.  A work
+// item (x = first entry of the key in ent_T/ent_S, y = entries, z = first
+// target proxy, w = first entry of the item) covers 128 target proxies and up
+// to 32 pairs.  Build sets the source count of the entries it takes to zero
+// in meta, so the regular far kernel skips them.
+struct CCGemm {
+    int n_items = 0;
+    long long shared = 0;  // entries taken
+    DBuf<int4> items;
+    DBuf<int> ent_T, ent_S;  // target and source panel of each keyed entry, in key order
+};
+constexpr int kCcMinGroup = 8;
+void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1, int min_group, int4 *meta,
+                  CCGemm &cg, hipStream_t s);
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1, const NodeTables &tabs,
+                   const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s);
+
 // Everything the interaction launches of one list need besides the charges,
 // the weights and the outputs: entry descriptors and the work items of the
 // near (PP+PC) and far (CP+CC) groups.  Built once (allocations, host
@@ -365,7 +437,9 @@ struct InteractPlan {
         bool mutual = false;
         Tasks tk;
         Mutual mu;
+        CCGemm cc;
     } g[2][4];  // [far][degree index]
+    long long cc_shared = 0;  // CC entries evaluated through shared kernel matrices
 };
 
 // Launch of one interaction group with everything fixed but the kernel.

@@ -108,6 +108,8 @@ void inclusive_sum_i64(const long long *in, long long *out, int n, hipStream_t s
 void inclusive_sum_i32(const int *in, int *out, int n, hipStream_t s);
 void exclusive_sum_i32(const int *in, int *out, int n, hipStream_t s);
 // Stable sort of (key, value) pairs by key; keys are < (1 << key_bits).
+void sort_pairs_u64(const unsigned long long *keys_in, unsigned long long *keys_out, const int *vals_in,
+                    int *vals_out, int n, int key_bits, hipStream_t s);
 void sort_pairs_i32(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int n, int key_bits,
                     hipStream_t s);
 
@@ -208,6 +210,7 @@ void gather_sorted(const Tree &t, const double *orig, double *sorted, hipStream_
 struct KernelEvents {
     hipEvent_t ev[4];
     bool near_done = false, far_done = false;
+    long long cc_shared = 0;  // CC entries the launches evaluated as shared-matrix products
 };
 // W and Psi hold every degree of the list (proxy_offset with 1 and C
 // components); with one degree they are the plain arrays.

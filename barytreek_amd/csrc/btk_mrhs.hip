@@ -1,6 +1,8 @@
-// Many charge vectors over one geometry (no reference counterpart; the
-// reference solves one charge vector per run).
+// MFMA proxy products: many charge vectors over one geometry, and CC pairs
+// sharing a kernel matrix (neither has a reference counterpart; the
+// reference solves one charge vector per run, pair by pair).
 //
+// Many charge vectors.
 // Tree, list, basis matrices and kernel evaluations are shared by all K
 // vectors, so the interactions become products G Q of the kernel tile G
 // (targets x sources) with the charge tile Q (sources x K):
@@ -211,6 +213,260 @@ __global__ void k_transpose_w(long long rows, int K, int kp, const double *__res
     for (int r = 0; r < kp; r++) Wt[i * kp + r] = r < K ? (float)W[(size_t)r * rows + i] : 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// CC pairs sharing a kernel matrix (CCGemm, btk_interact.hpp).
+// ---------------------------------------------------------------------------
+struct LevelBoxes {
+    const double *lo0, *lo1, *lo2, *hi0, *hi1, *hi2;
+    const int *level;
+};
+LevelBoxes level_boxes(const Tree &t) {
+    return LevelBoxes{t.lo[0].p, t.lo[1].p, t.lo[2].p, t.hi[0].p, t.hi[1].p, t.hi[2].p, t.level.p};
+}
+
+constexpr unsigned long long kNoKey = ~0ull;
+constexpr int kCcTgt = 128;  // target proxies per work item (32 per wavefront)
+constexpr int kCcCols = 32;  // pairs per work item
+constexpr int kKeyBits = 60;  // target level 6, source level 6, offset 3 x 16
+
+// Target panel of entry e of a group whose offsets are goff_g[0..pct].
+__device__ __forceinline__ int group_panel(const int *__restrict__ goff_g, int pct, int e) {
+    int lo = 0, hi = pct;  // goff_g[lo] <= e < goff_g[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (goff_g[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Key of entry e0 + i: levels and centre offset in half boxes of the finer
+// level; kNoKey for other types, taken entries, and offsets off the lattice.
+__global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct, const int4 *__restrict__ meta,
+                          LevelBoxes T, LevelBoxes S, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const int e = e0 + i;
+    vals[i] = e;
+    const int4 m = meta[e];
+    unsigned long long key = kNoKey;
+    if ((m.w & 3) == 3 && m.z > 0) {
+        const int t = group_panel(goff_g, pct, e), s = m.x;
+        const int lt = T.level[t], ls = S.level[s];
+        bool ok = lt < 64 && ls < 64;
+        unsigned long long k = ((unsigned long long)lt << 54) | ((unsigned long long)ls << 48);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double tl = sel3(d, T.lo0[t], T.lo1[t], T.lo2[t]), th = sel3(d, T.hi0[t], T.hi1[t], T.hi2[t]);
+            const double sl = sel3(d, S.lo0[s], S.lo1[s], S.lo2[s]), sh = sel3(d, S.hi0[s], S.hi1[s], S.hi2[s]);
+            const double h = 0.5 * fmin(th - tl, sh - sl);
+            const double off = 0.5 * (sl + sh) - 0.5 * (tl + th);
+            long long q = 0;
+            if (h > 0) {
+                const double r = off / h;
+                q = llrint(r);
+                ok = ok && fabs(r - (double)q) <= 1e-6 && q > -32768 && q < 32768;
+            } else {
+                ok = ok && off == 0.0;
+            }
+            k |= (unsigned long long)(q + 32768) << (32 - 16 * d);
+        }
+        if (ok) key = k;
+    }
+    keys[i] = key;
+}
+
+__global__ void k_cc_heads(int n, const unsigned long long *__restrict__ keys, int *__restrict__ head) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (keys[i] != kNoKey && (i == 0 || keys[i] != keys[i - 1])) ? 1 : 0;
+}
+
+// Start of each key (gid: inclusive count of heads) and the number of keyed
+// entries (they sort before the unkeyed ones).
+__global__ void k_cc_starts(int n, const unsigned long long *__restrict__ keys, const int *__restrict__ head,
+                            const int *__restrict__ gid, int *__restrict__ gstart, int *__restrict__ nvalid) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    if (head[i]) gstart[gid[i] - 1] = i;
+    if (keys[i] != kNoKey && (i + 1 == n || keys[i + 1] == kNoKey)) *nvalid = i + 1;
+}
+
+__global__ void k_cc_count(int G, int *__restrict__ gstart, const int *__restrict__ nvalid, int min_group, int ntt,
+                           int *__restrict__ nitems, unsigned long long *__restrict__ taken) {
+    const int g = blockIdx.x * TB + threadIdx.x;
+    if (g >= G) return;
+    const int end = g + 1 < G ? gstart[g + 1] : *nvalid;
+    const int size = end - gstart[g];
+    const bool sel = size >= min_group;
+    nitems[g] = sel ? ntt * ((size + kCcCols - 1) / kCcCols) : 0;
+    if (sel) atomicAdd(taken, (unsigned long long)size);
+}
+
+__global__ void k_cc_fill(int G, const int *__restrict__ gstart, const int *__restrict__ nvalid,
+                          const int *__restrict__ nitems, const int *__restrict__ ibase, int ntt,
+                          int4 *__restrict__ items) {
+    const int g = blockIdx.x;
+    const int ni = nitems[g];
+    if (ni == 0) return;
+    const int end = g + 1 < G ? gstart[g + 1] : *nvalid;
+    const int size = end - gstart[g];
+    for (int it = threadIdx.x; it < ni; it += blockDim.x)
+        items[ibase[g] + it] = make_int4(gstart[g], size, kCcTgt * (it % ntt), kCcCols * (it / ntt));
+}
+
+// Panels of every keyed entry in key order; the entries of the selected keys
+// leave the regular far kernel (source count zero in meta).
+__global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__restrict__ goff_g, int pct,
+                             const int *__restrict__ gid, const int *__restrict__ nitems, int4 *__restrict__ meta,
+                             int *__restrict__ entT, int *__restrict__ entS) {
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= nv) return;
+    const int e = vals[k];
+    entS[k] = meta[e].x;
+    entT[k] = group_panel(goff_g, pct, e);
+    if (nitems[gid[k] - 1] > 0) meta[e].z = 0;
+}
+
+// One work item: 128 target proxies x up to 32 pairs of one key, components
+// C0 .. C0+NC-1 of the kernel (the gradient kernel runs as two launches of
+// two components, which halves the accumulators and lets two wavefronts
+// share a SIMD).  Wave w takes targets 32*w .. +32 and all 32 pairs and
+// forms the transposed product, pairs x targets: the A operand holds the
+// pairs' proxy weights (FP32, from LDS, loaded one tile ahead into
+// registers), the B operand the kernel of the key's first pair (FP32
+// coordinates relative to its target box centre, as the far kernel's
+// TileRel) -- lane l evaluates it for target l%32 and source parity l/32.
+// FP32 tile products are added into FP64 sums; each accumulator row is one
+// pair, its lanes 32 consecutive target proxies, so the FP64 atomic adds
+// into each pair's own target cluster are coalesced.
+template <int KERNEL, int C0, int NC, bool EXCL>
+__global__ void __launch_bounds__(TB)
+    k_cc_gemm(const int4 *__restrict__ items, const int *__restrict__ entT, const int *__restrict__ entS,
+              LevelBoxes TB_, LevelBoxes SB, const double *__restrict__ W, int n1, NodeTables tabs, KParams kp,
+              double *__restrict__ out, size_t comp_stride) {
+    constexpr int C = KernelTraits<3, KERNEL>::C;
+    constexpr int QPT = 64 * kCcCols / TB;  // weights per thread per tile
+    static_assert(C0 + NC <= C, "component range");
+    __shared__ float sc[3][64];
+    __shared__ float qt[64][kCcCols + 1];
+    __shared__ double cn[3][kMaxN1];
+    __shared__ int sS[kCcCols], sT[kCcCols];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int4 it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
+    const int gs = it.x, gn = it.y, first = it.z, p0 = it.w;
+    const int T0 = entT[gs], S0 = entS[gs];
+    const int P = n1 * n1 * n1;
+    const double ctr[3] = {0.5 * (TB_.lo0[T0] + TB_.hi0[T0]), 0.5 * (TB_.lo1[T0] + TB_.hi1[T0]),
+                           0.5 * (TB_.lo2[T0] + TB_.hi2[T0])};
+    const int ti = first + 32 * w + (lane & 31);
+    double t[3];
+    proxy_point<3>(min(ti, P - 1), n1, tabs, TB_.lo0[T0], TB_.hi0[T0], TB_.lo1[T0], TB_.hi1[T0], TB_.lo2[T0],
+                   TB_.hi2[T0], t);
+    const float tr[3] = {(float)(t[0] - ctr[0]), (float)(t[1] - ctr[1]), (float)(t[2] - ctr[2])};
+    const int col = lane & 31;
+    const int par = lane >> 5;
+    if (tid < kCcCols) {
+        sS[tid] = (p0 + tid < gn) ? entS[gs + p0 + tid] : -1;
+        sT[tid] = (p0 + tid < gn) ? entT[gs + p0 + tid] : -1;
+    }
+    if (tid < 3 * n1) {
+        const int d = tid / n1, b = tid % n1;
+        cn[d][b] = proxy_node(tabs.b[b], sel3(d, SB.lo0[S0], SB.lo1[S0], SB.lo2[S0]),
+                              sel3(d, SB.hi0[S0], SB.hi1[S0], SB.hi2[S0]));
+    }
+    __syncthreads();
+    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64
+    auto load_q = [&](int t0, float *q) {
+#pragma unroll
+        for (int i = 0; i < QPT; i++) {
+            const int k = tid + TB * i;
+            const int s = k % 64, r = k / 64;
+            const int S = sS[r];
+            q[i] = (t0 + s < P && S >= 0) ? (float)W[(size_t)S * P + t0 + s] : 0.0f;
+        }
+    };
+    float qn[QPT];
+    load_q(0, qn);
+    double acc[NC][16];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[c][v] = 0.0;
+    const float eps2 = (float)kp.eps2;
+    for (int t0 = 0; t0 < P; t0 += 64) {
+        const int nt = min(64, P - t0);
+        if (tid < 64) {
+            const int j = min(t0 + tid, P - 1);
+            sc[0][tid] = (float)(cn[0][j / (n1 * n1)] - ctr[0]);
+            sc[1][tid] = (float)(cn[1][(j / n1) % n1] - ctr[1]);
+            sc[2][tid] = (float)(cn[2][j % n1] - ctr[2]);
+        }
+#pragma unroll
+        for (int i = 0; i < QPT; i++) {
+            const int k = tid + TB * i;
+            qt[k % 64][k / 64] = qn[i];
+        }
+        __syncthreads();
+        if (t0 + 64 < P) load_q(t0 + 64, qn);  // the next tile's weights, in flight during this one
+        f32x16 d[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) d[c] = (f32x16){0};
+#pragma unroll 4
+        for (int step = 0; step < 32; step++) {
+            const int s = 2 * step + par;
+            float g[C];
+            kernel_row<3, KERNEL, EXCL>(tr[0] - sc[0][s], tr[1] - sc[1][s], tr[2] - sc[2][s], eps2, g);
+            const float q = qt[s][col];
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, s < nt ? g[C0 + c] : 0.0f, d[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[c][v] += (double)d[c][v];
+        __syncthreads();
+    }
+    // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
+    if (ti >= P) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int Tc = sT[8 * (v / 4) + 4 * par + (v % 4)];
+        if (Tc < 0) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double scale = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc);
+            unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * acc[c][v]);
+        }
+    }
+}
+
+template <int KERNEL, int C0, int NC>
+void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1,
+                    const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
+    if (kp.excl)
+        k_cc_gemm<KERNEL, C0, NC, true><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p,
+                                                                   level_boxes(tt), level_boxes(ts), W, n1, tabs, kp,
+                                                                   Psi, comp_stride);
+    else
+        k_cc_gemm<KERNEL, C0, NC, false><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p,
+                                                                    level_boxes(tt), level_boxes(ts), W, n1, tabs,
+                                                                    kp, Psi, comp_stride);
+    BTK_LAUNCH_CHECK();
+}
+
+template <int KERNEL>
+void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1,
+                    const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
+    if constexpr (KernelTraits<3, KERNEL>::C == 1) {
+        launch_cc_part<KERNEL, 0, 1>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+    } else {
+        launch_cc_part<KERNEL, 0, 2>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_part<KERNEL, 2, 2>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+    }
+}
+
 }  // namespace
 
 // Declared in btk_internal.hpp.
@@ -321,5 +577,72 @@ template void launch_mrhs<3, BTK_KERNEL_POT_GRAD>(const TaskView &, int, const G
                                                    const SrcView &, const float *, const float *, double *, int, int,
                                                    int, const NodeTables &, const KParams &, int, int, int, int, bool,
                                                    bool, hipStream_t);
+
+
+void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1, int min_group, int4 *meta,
+                  CCGemm &cg, hipStream_t s) {
+    cg.n_items = 0;
+    cg.shared = 0;
+    const int pct = tt.pc;
+    const int *goff_g = l.goff.p + (size_t)grp * pct;
+    int *h = reinterpret_cast<int *>(host_scratch().v);
+    BTK_HIP(hipMemcpyAsync(h, goff_g, sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipMemcpyAsync(h + 1, goff_g + pct, sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    const int e0 = h[0], n = h[1] - h[0];
+    if (n <= 0) return;
+    DBuf<unsigned long long> keys(n), keys2(n);
+    DBuf<int> vals(n), vals2(n), head(n), gid(n);
+    k_cc_keys<<<nblk(n), TB, 0, s>>>(n, e0, goff_g, pct, meta, level_boxes(tt), level_boxes(ts), keys.p, vals.p);
+    BTK_LAUNCH_CHECK();
+    sort_pairs_u64(keys.p, keys2.p, vals.p, vals2.p, n, kKeyBits, s);
+    k_cc_heads<<<nblk(n), TB, 0, s>>>(n, keys2.p, head.p);
+    BTK_LAUNCH_CHECK();
+    inclusive_sum_i32(head.p, gid.p, n, s);
+    DBuf<int> gstart(n), nvalid(1);
+    BTK_HIP(hipMemsetAsync(nvalid.p, 0, sizeof(int), s));
+    k_cc_starts<<<nblk(n), TB, 0, s>>>(n, keys2.p, head.p, gid.p, gstart.p, nvalid.p);
+    BTK_LAUNCH_CHECK();
+    BTK_HIP(hipMemcpyAsync(h, gid.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipMemcpyAsync(h + 1, nvalid.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    const int G = h[0], nv = h[1];
+    if (G == 0) return;
+    const int P = n1 * n1 * n1;
+    const int ntt = (P + kCcTgt - 1) / kCcTgt;
+    DBuf<int> nitems(G), ibase(G);
+    DBuf<unsigned long long> taken(1);
+    BTK_HIP(hipMemsetAsync(taken.p, 0, sizeof(unsigned long long), s));
+    k_cc_count<<<nblk(G), TB, 0, s>>>(G, gstart.p, nvalid.p, min_group, ntt, nitems.p, taken.p);
+    BTK_LAUNCH_CHECK();
+    exclusive_sum_i32(nitems.p, ibase.p, G, s);
+    long long *hl = host_scratch().v + 2;
+    BTK_HIP(hipMemcpyAsync(h, ibase.p + (G - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipMemcpyAsync(h + 1, nitems.p + (G - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipMemcpyAsync(hl, taken.p, sizeof(long long), hipMemcpyDeviceToHost, s));
+    BTK_HIP(hipStreamSynchronize(s));
+    const int total = h[0] + h[1];
+    if (total == 0) return;
+    cg.shared = *hl;
+    cg.items.resize(total);
+    cg.ent_T.resize(nv);
+    cg.ent_S.resize(nv);
+    k_cc_fill<<<G, TB, 0, s>>>(G, gstart.p, nvalid.p, nitems.p, ibase.p, ntt, cg.items.p);
+    BTK_LAUNCH_CHECK();
+    k_cc_entries<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, gid.p, nitems.p, meta, cg.ent_T.p, cg.ent_S.p);
+    BTK_LAUNCH_CHECK();
+    cg.n_items = total;
+}
+
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1, const NodeTables &tabs,
+                   const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s) {
+    if (cg.n_items == 0) return;
+    if (kernel == BTK_KERNEL_POT)
+        launch_cc_gemm<BTK_KERNEL_POT>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+    else if (kernel == BTK_KERNEL_POT_GRAD)
+        launch_cc_gemm<BTK_KERNEL_POT_GRAD>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+    else
+        throw Error("shared-matrix CC products: potential or potential+gradient kernels");
+}
 
 }  // namespace btk

@@ -176,6 +176,16 @@ void sort_pairs_i32(const int *keys_in, int *keys_out, const int *vals_in, int *
     BTK_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys_in, keys_out, vals_in, vals_out, n, 0, key_bits, s));
 }
 
+void sort_pairs_u64(const unsigned long long *keys_in, unsigned long long *keys_out, const int *vals_in,
+                    int *vals_out, int n, int key_bits, hipStream_t s) {
+    if (n <= 0) return;
+    size_t tmp = 0;
+    BTK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys_in, keys_out, vals_in, vals_out, n, 0, key_bits,
+                                               s));
+    DBuf<char> t(tmp ? tmp : 1);
+    BTK_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys_in, keys_out, vals_in, vals_out, n, 0, key_bits, s));
+}
+
 // ---------------------------------------------------------------------------
 // Node tables: the same expressions as the reference, evaluated by host libm.
 // ---------------------------------------------------------------------------

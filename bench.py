@@ -196,6 +196,7 @@ def summarize(stats, ms, n, dtype, out, ds, flops_per_eval, idx=None):
                      "kernel": "k_near (PP+PC)", "flops_per_launch": near_flops, "avg_launch_s": t_near},
         "evals": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
         "pairs": [int(stats[-1].count[k]) for k in range(4)],
+        "cc_shared": int(stats[-1].cc_shared),
         "panels": int(stats[-1].panel_count_source), "levels": int(stats[-1].levels_source),
     }
     if ds is not None:

@@ -128,7 +128,10 @@ typedef struct btk_tuning {
     int far_tpt;    /* the same for CP+CC */
     int mutual;     /* 1 = mixed precision, one tree: a PP leaf pair listed in both orders, (T,S) and (S,T),
                        is evaluated once for both directions; 0 = once per entry (default) */
-    int reserved[3];
+    int cc_group;   /* mixed precision, 3D, potential (+gradient): CC pairs with the same two levels and
+                       box-centre offset share their kernel matrix; a set of at least cc_group such pairs
+                       runs as one FP32 MFMA product.  0 = default (8), -1 = off */
+    int reserved[2];
 } btk_tuning;
 int btk_set_tuning(const btk_tuning *t);
 int btk_get_tuning(btk_tuning *t);
@@ -239,6 +242,7 @@ typedef struct btk_solve_stats {
     double t_kernel_near; /* device time of the PP+PC kernel (HIP events on the solve's stream) */
     double t_kernel_far;  /* device time of the CP+CC kernel */
     int panel_count_target, panel_count_source, levels_target, levels_source;
+    long long cc_shared; /* CC pairs evaluated as shared-kernel-matrix products (btk_tuning.cc_group) */
 } btk_solve_stats;
 
 /* The driver's sequence (example_poisson_3d.cpp:70-160) as one call: tree(s),

@@ -35,7 +35,8 @@ def test_record_layouts_match_reference():
     assert RunConfig.fmm_theta.offset == 16 and RunConfig.ker_eps.offset == 48
     assert PANEL2D_DTYPE.itemsize == 80 and PANEL3D_DTYPE.itemsize == 112
     assert ctypes.sizeof(Options) == 32
-    assert ctypes.sizeof(SolveStats) == 6 * 8 + 4 * 8 + 4 * 8 + 2 * 8 + 4 * 4
+    assert ctypes.sizeof(SolveStats) == 6 * 8 + 4 * 8 + 4 * 8 + 2 * 8 + 4 * 4 + 8
+    assert SolveStats.cc_shared.offset == 144
     assert ctypes.sizeof(btk.Tuning) == 32
 
 

@@ -1,5 +1,7 @@
 """Launch shapes: the autotuner installs one of its candidates, and results
-do not depend on the shapes beyond the order of FP64 atomic adds."""
+do not depend on the shapes beyond the order of FP64 atomic adds; CC pairs
+that share a kernel matrix give the pairwise result when run as one MFMA
+product."""
 import numpy as np
 import pytest
 
@@ -16,7 +18,7 @@ ITEMS = {8192, 16384, 32768, 65536, 131072}
 def _restore_tuning():
     old = btk.get_tuning()
     yield
-    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual)
+    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual, old.cc_group)
 
 
 def _problem(dim, n, seed):
@@ -93,3 +95,61 @@ def test_mutual_pairs_match_one_way_evaluation(dim, kernel):
         rel = float(torch.linalg.norm(mutual - one_way) / torch.linalg.norm(one_way))
         assert rel < 1e-6, (tpt, rel)
         assert float(torch.linalg.norm(mutual - ref) / torch.linalg.norm(ref)) < 1e-6
+
+
+@pytest.mark.parametrize("kernel,degree,learned", [(btk.KERNEL_POT_GRAD, 8, False), (btk.KERNEL_POT, 6, False),
+                                                   (btk.KERNEL_POT_GRAD, 8, True)])
+def test_shared_matrix_cc_matches_pairwise(kernel, degree, learned):
+    """Mixed precision, 3D: the CC pairs grouped by (levels, centre offset)
+    and run as MFMA products against the same list evaluated pair by pair
+    (cc_group = -1), and both against the FP64 solve.  The grouped run must
+    take a large share of the CC pairs; the learned case mixes degrees 4, 6
+    and 8 in one list."""
+    print(f"\n[tuning] shared-matrix CC kernel={kernel} degree={degree} learned={learned}", flush=True)
+    n = 200000
+    d = _problem(3, n, 8)
+    cfg = btk.make_config(3, degree, 200, 0.7, 0.0)
+    C = btk.components(3, kernel)
+    old_w = None
+    if learned:  # a network whose predicted error depends on the separation only
+        from oracle import mlp
+        try:
+            old_w = btk.mac_get_weights()
+        except btk.BtkError:
+            pass
+        w = np.zeros(mlp.N_PARAMS, np.float32)
+        v = mlp.unpack(w)
+        v["W1"][0, 0] = 1.0
+        v["W2"][0, 0] = 1.0
+        for ty in range(3):
+            for k, c in enumerate((3.1, 3.7, 4.5, 5.3)):
+                v["W3"][ty * 4 + k, 0] = 4.0
+                v["b3"][ty * 4 + k] = -c
+        btk.mac_set_weights(w)
+        btk.mac_set_budget(1e-6)
+    outs, stats = {}, {}
+    for name, prec, cc in (("fp64", btk.PREC_FP64, 0), ("pairwise", btk.PREC_MIXED, -1),
+                           ("shared", btk.PREC_MIXED, 0), ("shared2", btk.PREC_MIXED, 2)):
+        btk.set_tuning(cc_group=cc)
+        o = btk.options(kernel=kernel, exclude_self=1, precision=prec, mac=int(learned))
+        out = torch.zeros(C * n, dtype=torch.float64, device="cuda")
+        stats[name] = btk.solve(cfg, o, d[0], d[1], d[2], d[3], out)
+        outs[name] = out
+    if learned:
+        btk.mac_set_budget(1e-7)
+        if old_w is not None:
+            btk.mac_set_weights(old_w)
+    cc_pairs = int(stats["shared"].count[3])
+    print(f"  CC pairs {cc_pairs}: shared {stats['shared'].cc_shared} (group >= 8), "
+          f"{stats['shared2'].cc_shared} (group >= 2)", flush=True)
+    assert stats["fp64"].cc_shared == 0 and stats["pairwise"].cc_shared == 0
+    assert stats["shared"].cc_shared >= 0.3 * cc_pairs
+    assert stats["shared2"].cc_shared >= stats["shared"].cc_shared
+    ref = outs["fp64"]
+    for name in ("pairwise", "shared", "shared2"):
+        rel = float(torch.linalg.norm(outs[name] - ref) / torch.linalg.norm(ref))
+        print(f"  {name}: rel vs FP64 {rel:.2e}", flush=True)
+        assert rel <= 1e-6
+    rel = float(torch.linalg.norm(outs["shared"] - outs["pairwise"]) / torch.linalg.norm(ref))
+    print(f"  shared vs pairwise {rel:.2e}", flush=True)
+    assert rel <= 5e-7
