@@ -265,7 +265,7 @@ __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct
             if (h > 0) {
                 const double r = off / h;
                 q = llrint(r);
-                ok = ok && fabs(r - (double)q) <= 1e-6 && q > -32768 && q < 32768;
+                ok = ok && fabs(r - (double)q) <= 1e-9 && q > -32768 && q < 32768;
             } else {
                 ok = ok && off == 0.0;
             }
@@ -292,13 +292,37 @@ __global__ void k_cc_starts(int n, const unsigned long long *__restrict__ keys, 
     if (keys[i] != kNoKey && (i + 1 == n || keys[i + 1] == kNoKey)) *nvalid = i + 1;
 }
 
-__global__ void k_cc_count(int G, int *__restrict__ gstart, const int *__restrict__ nvalid, int min_group, int ntt,
-                           int *__restrict__ nitems, unsigned long long *__restrict__ taken) {
+// A key fixes the geometry when every box of a level has one size, as in a
+// tree of one problem; each keyed entry still checks its boxes' extents
+// against the key's first pair and spoils the key (bad) on a mismatch.
+__global__ void k_cc_verify(int nv, const int *__restrict__ vals, const int *__restrict__ goff_g, int pct,
+                            const int4 *__restrict__ meta, const int *__restrict__ gid,
+                            const int *__restrict__ gstart, LevelBoxes T, LevelBoxes S, int *__restrict__ bad) {
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= nv) return;
+    const int g = gid[k] - 1;
+    const int e = vals[k], e0 = vals[gstart[g]];
+    const int t = group_panel(goff_g, pct, e), t0 = group_panel(goff_g, pct, e0);
+    const int s = meta[e].x, s0 = meta[e0].x;
+    bool same = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double a = sel3(d, T.hi0[t], T.hi1[t], T.hi2[t]) - sel3(d, T.lo0[t], T.lo1[t], T.lo2[t]);
+        const double a0 = sel3(d, T.hi0[t0], T.hi1[t0], T.hi2[t0]) - sel3(d, T.lo0[t0], T.lo1[t0], T.lo2[t0]);
+        const double b = sel3(d, S.hi0[s], S.hi1[s], S.hi2[s]) - sel3(d, S.lo0[s], S.lo1[s], S.lo2[s]);
+        const double b0 = sel3(d, S.hi0[s0], S.hi1[s0], S.hi2[s0]) - sel3(d, S.lo0[s0], S.lo1[s0], S.lo2[s0]);
+        same = same && fabs(a - a0) <= 1e-9 * fabs(a0) && fabs(b - b0) <= 1e-9 * fabs(b0);
+    }
+    if (!same) bad[g] = 1;
+}
+
+__global__ void k_cc_count(int G, int *__restrict__ gstart, const int *__restrict__ nvalid, const int *__restrict__ bad,
+                           int min_group, int ntt, int *__restrict__ nitems, unsigned long long *__restrict__ taken) {
     const int g = blockIdx.x * TB + threadIdx.x;
     if (g >= G) return;
     const int end = g + 1 < G ? gstart[g + 1] : *nvalid;
     const int size = end - gstart[g];
-    const bool sel = size >= min_group;
+    const bool sel = size >= min_group && !bad[g];
     nitems[g] = sel ? ntt * ((size + kCcCols - 1) / kCcCols) : 0;
     if (sel) atomicAdd(taken, (unsigned long long)size);
 }
@@ -583,6 +607,8 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
                   CCGemm &cg, hipStream_t s) {
     cg.n_items = 0;
     cg.shared = 0;
+    // a forest's problems have boxes of their own sizes at every level
+    if (tt.nprob > 1 || ts.nprob > 1) return;
     const int pct = tt.pc;
     const int *goff_g = l.goff.p + (size_t)grp * pct;
     int *h = reinterpret_cast<int *>(host_scratch().v);
@@ -610,10 +636,14 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
     if (G == 0) return;
     const int P = n1 * n1 * n1;
     const int ntt = (P + kCcTgt - 1) / kCcTgt;
-    DBuf<int> nitems(G), ibase(G);
+    DBuf<int> nitems(G), ibase(G), bad(G);
     DBuf<unsigned long long> taken(1);
     BTK_HIP(hipMemsetAsync(taken.p, 0, sizeof(unsigned long long), s));
-    k_cc_count<<<nblk(G), TB, 0, s>>>(G, gstart.p, nvalid.p, min_group, ntt, nitems.p, taken.p);
+    BTK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int) * G, s));
+    k_cc_verify<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, meta, gid.p, gstart.p, level_boxes(tt),
+                                        level_boxes(ts), bad.p);
+    BTK_LAUNCH_CHECK();
+    k_cc_count<<<nblk(G), TB, 0, s>>>(G, gstart.p, nvalid.p, bad.p, min_group, ntt, nitems.p, taken.p);
     BTK_LAUNCH_CHECK();
     exclusive_sum_i32(nitems.p, ibase.p, G, s);
     long long *hl = host_scratch().v + 2;

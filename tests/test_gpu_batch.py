@@ -63,3 +63,35 @@ def test_batch_trees_are_the_individual_trees():
         t = btk.Tree(cfg, *[torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, y, z)])
         total += t.panel_count
     assert st.panel_count_source == total
+
+
+def test_batch_mixed_with_many_cc_pairs():
+    """Mixed precision, pot+grad: a forest of problems whose boxes differ in
+    size, each with enough CC pairs for the shared-kernel-matrix products of
+    a single-problem solve; every problem within 1e-6 of its FP64 solve."""
+    print("\n[batch] mixed, large problems of different extents", flush=True)
+    sizes = [60000, 150000, 90000, 120000]
+    pts = []
+    for i, m in enumerate(sizes):
+        x, y, z, q = wl.plummer_3d(m, 300 + i, a=0.5, rmax=5.0)
+        sc = 0.5 + 0.7 * i  # every problem has its own root box
+        pts.append((x * sc, y * sc, z * sc, q))
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    cat = [torch.from_numpy(np.concatenate([p[k] for p in pts])).cuda() for k in range(4)]
+    cfg = btk.make_config(3, 6, 200, 0.7, 0.0)
+    C = btk.components(3, btk.KERNEL_POT_GRAD)
+    out = torch.zeros(C * off[-1], dtype=torch.float64, device="cuda")
+    btk.solve_batch(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=btk.PREC_MIXED), off,
+                    cat[0], cat[1], cat[2], cat[3], out)
+    got = out.view(C, -1).cpu().numpy()
+    worst = 0.0
+    for i, (x, y, z, q) in enumerate(pts):
+        m = sizes[i]
+        dd = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, y, z, q)]
+        ref = torch.zeros(C * m, dtype=torch.float64, device="cuda")
+        st = btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1), dd[0], dd[1], dd[2], dd[3], ref)
+        ref = ref.view(C, m).cpu().numpy()
+        e = np.linalg.norm(got[:, off[i]:off[i + 1]] - ref) / np.linalg.norm(ref)
+        print(f"  problem {i}: {m} points, {int(st.count[3])} CC pairs, rel vs FP64 {e:.2e}", flush=True)
+        worst = max(worst, e)
+    assert worst < 1e-6
