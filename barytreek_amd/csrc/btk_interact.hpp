@@ -393,13 +393,13 @@ extern template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, cons
 extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
                                                            const KParams &, double *, hipStream_t);
 
-// CC pairs sharing a kernel matrix (btk_mrhs.hip), mixed precision, 3D, one
-// problem per tree.  Panel boxes are exact halves of their parents, so the
-// P x P kernel matrix of a CC pair depends only on the two levels and the
-// offset of the box centres (a multiple of half the finer box per axis when
-// targets and sources share the root box; keys are checked against the
-// boxes' extents).  The CC entries of one far
-// group are keyed by (target level, source level, offset) and every key held
+// CC pairs sharing a kernel matrix (btk_mrhs.hip), mixed precision, 3D.
+// Panel boxes are exact halves of their parents, so the P x P kernel matrix
+// of a CC pair depends only on the problem (a forest's problems have roots of
+// their own), the two levels and the offset of the box centres (a multiple of
+// half the finer box per axis when targets and sources share the root box;
+// keys are checked against the boxes' extents).  The CC entries of one far
+// group are keyed by (problem, target level, source level, offset) and every key held
 // by at least min_group entries becomes one MFMA product: the matrix (from
 // the key's first pair) times the pairs' proxy weights as columns.  A work
 // item (x = first entry of the key in ent_T/ent_S, y = entries, z = first

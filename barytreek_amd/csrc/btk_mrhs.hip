@@ -227,7 +227,6 @@ LevelBoxes level_boxes(const Tree &t) {
 constexpr unsigned long long kNoKey = ~0ull;
 constexpr int kCcTgt = 128;  // target proxies per work item (32 per wavefront)
 constexpr int kCcCols = 32;  // pairs per work item
-constexpr int kKeyBits = 60;  // target level 6, source level 6, offset 3 x 16
 
 // Target panel of entry e of a group whose offsets are goff_g[0..pct].
 __device__ __forceinline__ int group_panel(const int *__restrict__ goff_g, int pct, int e) {
@@ -240,10 +239,17 @@ __device__ __forceinline__ int group_panel(const int *__restrict__ goff_g, int p
     return lo;
 }
 
-// Key of entry e0 + i: levels and centre offset in half boxes of the finer
-// level; kNoKey for other types, taken entries, and offsets off the lattice.
+// Key of entry e0 + i: problem (forests: the target panel's root), levels,
+// and centre offset in half boxes of the finer level, ob bits per axis; the
+// key's width is pbits + 12 + 3 ob <= 63 bits.  kNoKey for other types, taken
+// entries, and offsets off the lattice or out of range.
+struct KeyLayout {
+    int nprob, pbits, ob;
+};
+
 __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct, const int4 *__restrict__ meta,
-                          LevelBoxes T, LevelBoxes S, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+                          LevelBoxes T, LevelBoxes S, const int *__restrict__ tparent, KeyLayout kl,
+                          unsigned long long *__restrict__ keys, int *__restrict__ vals) {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const int e = e0 + i;
@@ -253,8 +259,11 @@ __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct
     if ((m.w & 3) == 3 && m.z > 0) {
         const int t = group_panel(goff_g, pct, e), s = m.x;
         const int lt = T.level[t], ls = S.level[s];
-        bool ok = lt < 64 && ls < 64;
-        unsigned long long k = ((unsigned long long)lt << 54) | ((unsigned long long)ls << 48);
+        int root = t;
+        while (root >= kl.nprob) root = tparent[root];
+        const long long half = 1ll << (kl.ob - 1);
+        bool ok = lt < 63 && ls < 63;  // (a key of all ones would tie with kNoKey)
+        unsigned long long k = ((((unsigned long long)root << 6) | (unsigned long long)lt) << 6 | ls) << (3 * kl.ob);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double tl = sel3(d, T.lo0[t], T.lo1[t], T.lo2[t]), th = sel3(d, T.hi0[t], T.hi1[t], T.hi2[t]);
@@ -265,11 +274,11 @@ __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct
             if (h > 0) {
                 const double r = off / h;
                 q = llrint(r);
-                ok = ok && fabs(r - (double)q) <= 1e-9 && q > -32768 && q < 32768;
+                ok = ok && fabs(r - (double)q) <= 1e-9 && q > -half && q < half;
             } else {
                 ok = ok && off == 0.0;
             }
-            k |= (unsigned long long)(q + 32768) << (32 - 16 * d);
+            k |= (unsigned long long)(q + half) << (kl.ob * (2 - d));
         }
         if (ok) key = k;
     }
@@ -607,8 +616,14 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
                   CCGemm &cg, hipStream_t s) {
     cg.n_items = 0;
     cg.shared = 0;
-    // a forest's problems have boxes of their own sizes at every level
-    if (tt.nprob > 1 || ts.nprob > 1) return;
+    // a forest's problems have boxes of their own sizes at every level: the
+    // key leads with the problem (one tree for targets and sources)
+    if ((tt.nprob > 1 || ts.nprob > 1) && &tt != &ts) return;
+    KeyLayout kl{tt.nprob, 0, 16};
+    while ((1ll << kl.pbits) < tt.nprob) kl.pbits++;
+    kl.ob = std::min(16, (63 - 12 - kl.pbits) / 3);
+    if (kl.ob < 6) return;
+    const int key_bits = kl.pbits + 12 + 3 * kl.ob;
     const int pct = tt.pc;
     const int *goff_g = l.goff.p + (size_t)grp * pct;
     int *h = reinterpret_cast<int *>(host_scratch().v);
@@ -619,9 +634,10 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
     if (n <= 0) return;
     DBuf<unsigned long long> keys(n), keys2(n);
     DBuf<int> vals(n), vals2(n), head(n), gid(n);
-    k_cc_keys<<<nblk(n), TB, 0, s>>>(n, e0, goff_g, pct, meta, level_boxes(tt), level_boxes(ts), keys.p, vals.p);
+    k_cc_keys<<<nblk(n), TB, 0, s>>>(n, e0, goff_g, pct, meta, level_boxes(tt), level_boxes(ts), tt.parent.p, kl,
+                                     keys.p, vals.p);
     BTK_LAUNCH_CHECK();
-    sort_pairs_u64(keys.p, keys2.p, vals.p, vals2.p, n, kKeyBits, s);
+    sort_pairs_u64(keys.p, keys2.p, vals.p, vals2.p, n, key_bits, s);
     k_cc_heads<<<nblk(n), TB, 0, s>>>(n, keys2.p, head.p);
     BTK_LAUNCH_CHECK();
     inclusive_sum_i32(head.p, gid.p, n, s);

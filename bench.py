@@ -101,28 +101,30 @@ MAC_BUDGETS = (1e-5, 8e-6, 6e-6, 5e-6, 4e-6, 3.5e-6, 3e-6, 2e-6, 1e-6, 3e-7)
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta):
-    """Learned MAC for 3D Plummer-like sets: samples from the treecode's own
-    runs on three 1M-point Plummer sets (a = 0.3, other seeds than the
-    workload's), 10000 pairs each, then 200 epochs of Adam; returns the
-    training record."""
+    """Learned MAC for 3D Plummer-like sets: 1M cluster pairs sampled from the
+    treecode's own runs on ten 2M-point Plummer sets (a = 0.3, other seeds
+    than the workload's), 100000 pairs each, then 20 epochs of Adam on
+    minibatches of 4096; returns the training record."""
     cfg = btk.make_config(3, degree, leaf, theta, 0.0)
     Xs, Ys, total = [], [], 0
     t0 = time.perf_counter()
-    for seed in (11, 12, 13):
-        x, y, z, _ = wl.plummer_3d(1_000_000, seed, a=0.3, rmax=10.0)
+    per = 100_000
+    for seed in range(11, 21):
+        x, y, z, _ = wl.plummer_3d(2_000_000, seed, a=0.3, rmax=10.0)
         d = [torch.from_numpy(a).cuda() for a in (x, y, z)]
-        X = torch.zeros(10000 * 10, dtype=torch.float32, device="cuda")
-        Y = torch.zeros(10000 * 12, dtype=torch.float32, device="cuda")
-        m = btk.mac_sample(cfg, d[0], d[1], d[2], 10000, seed, X, Y)
+        X = torch.zeros(per * 10, dtype=torch.float32, device="cuda")
+        Y = torch.zeros(per * 12, dtype=torch.float32, device="cuda")
+        m = btk.mac_sample(cfg, d[0], d[1], d[2], per, seed, X, Y)
         Xs.append(X[:m * 10])
         Ys.append(Y[:m * 12])
         total += m
+        del d
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    loss = btk.mac_train(torch.cat(Xs), torch.cat(Ys), total, epochs=200, batch=1024, lr=2e-3, seed=3, reinit=True)
+    loss = btk.mac_train(torch.cat(Xs), torch.cat(Ys), total, epochs=20, batch=4096, lr=2e-3, seed=3, reinit=True)
     torch.cuda.synchronize()
     return {"samples": total, "sample_s": t1 - t0, "train_s": time.perf_counter() - t1, "loss": loss,
-            "sets": "3 Plummer sets of 1M points (a=0.3, seeds 11-13)"}
+            "sets": "10 Plummer sets of 2M points (a=0.3, seeds 11-20), 100000 pairs each"}
 
 
 def log(msg):

@@ -67,8 +67,9 @@ def test_batch_trees_are_the_individual_trees():
 
 def test_batch_mixed_with_many_cc_pairs():
     """Mixed precision, pot+grad: a forest of problems whose boxes differ in
-    size, each with enough CC pairs for the shared-kernel-matrix products of
-    a single-problem solve; every problem within 1e-6 of its FP64 solve."""
+    size, each with enough CC pairs for the shared-kernel-matrix products
+    (keyed per problem, so no key mixes box sizes); every problem within 1e-6
+    of its FP64 solve."""
     print("\n[batch] mixed, large problems of different extents", flush=True)
     sizes = [60000, 150000, 90000, 120000]
     pts = []
@@ -81,8 +82,10 @@ def test_batch_mixed_with_many_cc_pairs():
     cfg = btk.make_config(3, 6, 200, 0.7, 0.0)
     C = btk.components(3, btk.KERNEL_POT_GRAD)
     out = torch.zeros(C * off[-1], dtype=torch.float64, device="cuda")
-    btk.solve_batch(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=btk.PREC_MIXED), off,
-                    cat[0], cat[1], cat[2], cat[3], out)
+    sb = btk.solve_batch(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=btk.PREC_MIXED),
+                         off, cat[0], cat[1], cat[2], cat[3], out)
+    print(f"  forest: {int(sb.count[3])} CC pairs, {int(sb.cc_shared)} through shared kernel matrices", flush=True)
+    assert sb.cc_shared > 0.3 * sb.count[3]  # keys per problem: the forest shares matrices too
     got = out.view(C, -1).cpu().numpy()
     worst = 0.0
     for i, (x, y, z, q) in enumerate(pts):
