@@ -330,8 +330,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             // mixed precision, 3D: CC pairs that share a kernel matrix run as
             // MFMA products
             if (far && f32 && dim == 3 && o.kernel != BTK_KERNEL_REF_VEL3D && tu.cc_group >= 0) {
-                build_ccgemm(l, tt, ts, grp, l.deg[di] + 1, tu.cc_group > 0 ? tu.cc_group : kCcMinGroup,
-                             p->meta.p, g.cc, s);
+                build_ccgemm(l, tt, ts, grp, tu.cc_group > 0 ? tu.cc_group : kCcMinGroup, p->meta.p, g.cc, s);
                 p->cc_shared += g.cc.shared;
             }
             const int4 *meta = (g.mutual || g.cc.n_items > 0) ? p->meta.p : nullptr;

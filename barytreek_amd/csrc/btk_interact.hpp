@@ -403,8 +403,8 @@ extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const
 // by at least min_group entries becomes one MFMA product: the matrix (from
 // the key's first pair) times the pairs' proxy weights as columns.  A work
 // item (x = first entry of the key in ent_T/ent_S, y = entries, z = first
-// target proxy, w = first entry of the item) covers 128 target proxies and up
-// to 32 pairs.  Build sets the source count of the entries it takes to zero
+// target proxy (0), w = first entry of the item) covers up to 32 pairs
+// against all target proxies, 128 at a time.  Build sets the source count of the entries it takes to zero
 // in meta, so the regular far kernel skips them.
 struct CCGemm {
     int n_items = 0;
@@ -413,7 +413,7 @@ struct CCGemm {
     DBuf<int> ent_T, ent_S;  // target and source panel of each keyed entry, in key order
 };
 constexpr int kCcMinGroup = 8;
-void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1, int min_group, int4 *meta,
+void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,
                   CCGemm &cg, hipStream_t s);
 void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1, const NodeTables &tabs,
                    const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s);

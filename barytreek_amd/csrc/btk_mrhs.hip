@@ -326,18 +326,18 @@ __global__ void k_cc_verify(int nv, const int *__restrict__ vals, const int *__r
 }
 
 __global__ void k_cc_count(int G, int *__restrict__ gstart, const int *__restrict__ nvalid, const int *__restrict__ bad,
-                           int min_group, int ntt, int *__restrict__ nitems, unsigned long long *__restrict__ taken) {
+                           int min_group, int *__restrict__ nitems, unsigned long long *__restrict__ taken) {
     const int g = blockIdx.x * TB + threadIdx.x;
     if (g >= G) return;
     const int end = g + 1 < G ? gstart[g + 1] : *nvalid;
     const int size = end - gstart[g];
     const bool sel = size >= min_group && !bad[g];
-    nitems[g] = sel ? ntt * ((size + kCcCols - 1) / kCcCols) : 0;
+    nitems[g] = sel ? (size + kCcCols - 1) / kCcCols : 0;
     if (sel) atomicAdd(taken, (unsigned long long)size);
 }
 
 __global__ void k_cc_fill(int G, const int *__restrict__ gstart, const int *__restrict__ nvalid,
-                          const int *__restrict__ nitems, const int *__restrict__ ibase, int ntt,
+                          const int *__restrict__ nitems, const int *__restrict__ ibase,
                           int4 *__restrict__ items) {
     const int g = blockIdx.x;
     const int ni = nitems[g];
@@ -345,7 +345,7 @@ __global__ void k_cc_fill(int G, const int *__restrict__ gstart, const int *__re
     const int end = g + 1 < G ? gstart[g + 1] : *nvalid;
     const int size = end - gstart[g];
     for (int it = threadIdx.x; it < ni; it += blockDim.x)
-        items[ibase[g] + it] = make_int4(gstart[g], size, kCcTgt * (it % ntt), kCcCols * (it / ntt));
+        items[ibase[g] + it] = make_int4(gstart[g], size, 0, kCcCols * it);
 }
 
 // Panels of every keyed entry in key order; the entries of the selected keys
@@ -387,16 +387,11 @@ __global__ void __launch_bounds__(TB)
     __shared__ int sS[kCcCols], sT[kCcCols];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int4 it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
-    const int gs = it.x, gn = it.y, first = it.z, p0 = it.w;
+    const int gs = it.x, gn = it.y, p0 = it.w;
     const int T0 = entT[gs], S0 = entS[gs];
     const int P = n1 * n1 * n1;
     const double ctr[3] = {0.5 * (TB_.lo0[T0] + TB_.hi0[T0]), 0.5 * (TB_.lo1[T0] + TB_.hi1[T0]),
                            0.5 * (TB_.lo2[T0] + TB_.hi2[T0])};
-    const int ti = first + 32 * w + (lane & 31);
-    double t[3];
-    proxy_point<3>(min(ti, P - 1), n1, tabs, TB_.lo0[T0], TB_.hi0[T0], TB_.lo1[T0], TB_.hi1[T0], TB_.lo2[T0],
-                   TB_.hi2[T0], t);
-    const float tr[3] = {(float)(t[0] - ctr[0]), (float)(t[1] - ctr[1]), (float)(t[2] - ctr[2])};
     const int col = lane & 31;
     const int par = lane >> 5;
     if (tid < kCcCols) {
@@ -419,15 +414,31 @@ __global__ void __launch_bounds__(TB)
             q[i] = (t0 + s < P && S >= 0) ? (float)W[(size_t)S * P + t0 + s] : 0.0f;
         }
     };
+    // the item runs every block of kCcTgt targets against every 64-source
+    // tile, one sequence of (target block, source tile) steps; the weights
+    // of the next step are loaded during the current one
+    const int nst = (P + 63) / 64;
+    const int steps = nst * ((P + kCcTgt - 1) / kCcTgt);
     float qn[QPT];
     load_q(0, qn);
     double acc[NC][16];
-#pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int v = 0; v < 16; v++) acc[c][v] = 0.0;
+    float tr[3] = {0.0f, 0.0f, 0.0f};
+    int ti = 0;
     const float eps2 = (float)kp.eps2;
-    for (int t0 = 0; t0 < P; t0 += 64) {
+    for (int st = 0; st < steps; st++) {
+        const int t0 = (st % nst) * 64;
+        if (t0 == 0) {  // a new block of targets
+            ti = (st / nst) * kCcTgt + 32 * w + (lane & 31);
+            double t[3];
+            proxy_point<3>(min(ti, P - 1), n1, tabs, TB_.lo0[T0], TB_.hi0[T0], TB_.lo1[T0], TB_.hi1[T0],
+                           TB_.lo2[T0], TB_.hi2[T0], t);
+#pragma unroll
+            for (int d = 0; d < 3; d++) tr[d] = (float)(t[d] - ctr[d]);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[c][v] = 0.0;
+        }
         const int nt = min(64, P - t0);
         if (tid < 64) {
             const int j = min(t0 + tid, P - 1);
@@ -441,7 +452,7 @@ __global__ void __launch_bounds__(TB)
             qt[k % 64][k / 64] = qn[i];
         }
         __syncthreads();
-        if (t0 + 64 < P) load_q(t0 + 64, qn);  // the next tile's weights, in flight during this one
+        if (st + 1 < steps) load_q(((st + 1) % nst) * 64, qn);
         f32x16 d[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) d[c] = (f32x16){0};
@@ -460,17 +471,18 @@ __global__ void __launch_bounds__(TB)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[c][v] += (double)d[c][v];
         __syncthreads();
-    }
-    // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
-    if (ti >= P) return;
+        // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
+        if (t0 + 64 >= P && ti < P) {  // the block's last source tile: its sums are complete
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int Tc = sT[8 * (v / 4) + 4 * par + (v % 4)];
-        if (Tc < 0) continue;
+            for (int v = 0; v < 16; v++) {
+                const int Tc = sT[8 * (v / 4) + 4 * par + (v % 4)];
+                if (Tc < 0) continue;
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            const double scale = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc);
-            unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * acc[c][v]);
+                for (int c = 0; c < NC; c++) {
+                    const double scale = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc);
+                    unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * acc[c][v]);
+                }
+            }
         }
     }
 }
@@ -612,7 +624,7 @@ template void launch_mrhs<3, BTK_KERNEL_POT_GRAD>(const TaskView &, int, const G
                                                    bool, hipStream_t);
 
 
-void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1, int min_group, int4 *meta,
+void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,
                   CCGemm &cg, hipStream_t s) {
     cg.n_items = 0;
     cg.shared = 0;
@@ -650,8 +662,6 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
     BTK_HIP(hipStreamSynchronize(s));
     const int G = h[0], nv = h[1];
     if (G == 0) return;
-    const int P = n1 * n1 * n1;
-    const int ntt = (P + kCcTgt - 1) / kCcTgt;
     DBuf<int> nitems(G), ibase(G), bad(G);
     DBuf<unsigned long long> taken(1);
     BTK_HIP(hipMemsetAsync(taken.p, 0, sizeof(unsigned long long), s));
@@ -659,7 +669,7 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
     k_cc_verify<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, meta, gid.p, gstart.p, level_boxes(tt),
                                         level_boxes(ts), bad.p);
     BTK_LAUNCH_CHECK();
-    k_cc_count<<<nblk(G), TB, 0, s>>>(G, gstart.p, nvalid.p, bad.p, min_group, ntt, nitems.p, taken.p);
+    k_cc_count<<<nblk(G), TB, 0, s>>>(G, gstart.p, nvalid.p, bad.p, min_group, nitems.p, taken.p);
     BTK_LAUNCH_CHECK();
     exclusive_sum_i32(nitems.p, ibase.p, G, s);
     long long *hl = host_scratch().v + 2;
@@ -673,7 +683,7 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int n1
     cg.items.resize(total);
     cg.ent_T.resize(nv);
     cg.ent_S.resize(nv);
-    k_cc_fill<<<G, TB, 0, s>>>(G, gstart.p, nvalid.p, nitems.p, ibase.p, ntt, cg.items.p);
+    k_cc_fill<<<G, TB, 0, s>>>(G, gstart.p, nvalid.p, nitems.p, ibase.p, cg.items.p);
     BTK_LAUNCH_CHECK();
     k_cc_entries<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, gid.p, nitems.p, meta, cg.ent_T.p, cg.ent_S.p);
     BTK_LAUNCH_CHECK();
