@@ -97,7 +97,7 @@ def parse():
 
 
 # per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (1e-5, 8e-6, 6e-6, 5e-6, 4e-6, 3.5e-6, 3e-6, 2e-6, 1e-6, 3e-7)
+MAC_BUDGETS = (1e-5, 8e-6, 6e-6, 5e-6, 4.5e-6, 4e-6, 3.5e-6, 3.25e-6, 3e-6, 2.5e-6, 2e-6, 1e-6, 3e-7)
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta):
