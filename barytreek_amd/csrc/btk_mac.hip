@@ -7,11 +7,13 @@
 // point index.  For degrees 4, 6, 8 and 10 the kernel forms the PC
 // (targets from S's proxy weights), CP (S's points to T's proxies, then
 // interpolation to the targets) and CC (S's proxy weights to T's proxies,
-// then interpolation) approximations of the 1/r potential exactly as the
-// treecode does (barycentric basis on the Chebyshev-Lobatto nodes, proxy
-// points cos(pi/n*i)), and their relative L2 error against the FP64 direct
-// sum over the same points: 12 targets per sample, stored as log10 / 4 with
-// log10 clamped to [-14, 0].
+// then interpolation) approximations of the 1/r potential and its gradient
+// kernel d/r^3 exactly as the treecode does (barycentric basis on the
+// Chebyshev-Lobatto nodes, proxy points cos(pi/n*i), every component
+// interpolated on its own), and their relative L2 error over the four
+// components at the targets against the FP64 direct sum over the same
+// points: 12 targets per sample, stored as log10 / 4 with log10 clamped to
+// [-14, 0].
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -47,20 +49,28 @@ __device__ __forceinline__ double hash_charge(unsigned seed, int i) {
     return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-__device__ __forceinline__ double g_pot(double dx, double dy, double dz) {
-    return rsqrt_f64(dx * dx + dy * dy + dz * dz);
+constexpr int kSampC = 4;  // potential and gradient components
+
+// 1/r and d/r^3 (the potential+gradient kernel's components).
+__device__ __forceinline__ void g_all(double dx, double dy, double dz, double *g) {
+    const double inv = rsqrt_f64(dx * dx + dy * dy + dz * dz);
+    const double h = inv * inv * inv;
+    g[0] = inv;
+    g[1] = h * dx;
+    g[2] = h * dy;
+    g[3] = h * dz;
 }
 
 // Shared memory of one sample (one workgroup).
 struct SampleLds {
-    double W[kMaxN1 * kMaxN1 * kMaxN1];    // S's proxy weights
-    double Psi[kMaxN1 * kMaxN1 * kMaxN1];  // T's proxy potentials
-    double LB[kSampT][3][kMaxN1];          // basis of a chunk of 64 sources
+    double W[kMaxN1 * kMaxN1 * kMaxN1];            // S's proxy weights
+    double Psi[kSampC][kMaxN1 * kMaxN1 * kMaxN1];  // T's proxy potentials, per component
+    double LB[kSampT][3][kMaxN1];                  // basis of a chunk of 64 sources
     double LQ[kSampT];
-    double LT[kSampT][3][kMaxN1];          // basis of the targets
-    double tx[kSampT], ty[kSampT], tz[kSampT], ex[kSampT];
-    double ap[3][kSampT];                  // PC, CP, CC at the targets
-    double red[4][kSampT];
+    double LT[kSampT][3][kMaxN1];                  // basis of the targets
+    double tx[kSampT], ty[kSampT], tz[kSampT], ex[kSampC][kSampT];
+    double ap[3][kSampC][kSampT];                  // PC, CP, CC at the targets
+    double red[4][kSampC][kSampT];
 };
 
 // Sampled point k of m out of a panel's cnt points.
@@ -114,56 +124,63 @@ __device__ void sample_degree(const SampleTree &tr, int T, int S, int nT, int nS
     __syncthreads();
     // PC: targets from S's proxy points and weights
     {
-        double a = 0.0;
+        double a[kSampC] = {0.0, 0.0, 0.0, 0.0};
         if (t < nT) {
             for (int l = part; l < P; l += 4) {
                 const int m = l % N1, k = (l / N1) % N1, j = l / (N1 * N1);
-                a = fma(g_pot(L.tx[t] - proxy_node(tab.b[j], sl[0], sh[0]),
-                              L.ty[t] - proxy_node(tab.b[k], sl[1], sh[1]),
-                              L.tz[t] - proxy_node(tab.b[m], sl[2], sh[2])),
-                        L.W[l], a);
+                double g[kSampC];
+                g_all(L.tx[t] - proxy_node(tab.b[j], sl[0], sh[0]), L.ty[t] - proxy_node(tab.b[k], sl[1], sh[1]),
+                      L.tz[t] - proxy_node(tab.b[m], sl[2], sh[2]), g);
+                for (int c = 0; c < kSampC; c++) a[c] = fma(g[c], L.W[l], a[c]);
             }
         }
-        L.red[part][t] = a;
+        for (int c = 0; c < kSampC; c++) L.red[part][c][t] = a[c];
         __syncthreads();
-        if (tid < nT) L.ap[0][tid] = L.red[0][tid] + L.red[1][tid] + L.red[2][tid] + L.red[3][tid];
+        if (tid < nT)
+            for (int c = 0; c < kSampC; c++)
+                L.ap[0][c][tid] = L.red[0][c][tid] + L.red[1][c][tid] + L.red[2][c][tid] + L.red[3][c][tid];
         __syncthreads();
     }
-    // CP and CC: T's proxy potentials from S's points (CP) or proxies (CC),
-    // then interpolation to the targets (leaf_to_point)
+    // CP and CC: T's proxy values from S's points (CP) or proxies (CC), then
+    // interpolation of each component to the targets (leaf_to_point)
     for (int kind = 1; kind <= 2; kind++) {
         for (int k = tid; k < P; k += blockDim.x) {
             const int m = k % N1, kk = (k / N1) % N1, j = k / (N1 * N1);
             const double px = proxy_node(tab.b[j], tl[0], th[0]);
             const double py = proxy_node(tab.b[kk], tl[1], th[1]);
             const double pz = proxy_node(tab.b[m], tl[2], th[2]);
-            double a = 0.0;
+            double a[kSampC] = {0.0, 0.0, 0.0, 0.0}, g[kSampC];
             if (kind == 1) {
                 for (int s = 0; s < nS; s++) {
                     const int i = pick(tr.start[S], cS, nS, s);
-                    a = fma(g_pot(px - tr.sx[i], py - tr.sy[i], pz - tr.sz[i]), hash_charge(seed, i), a);
+                    g_all(px - tr.sx[i], py - tr.sy[i], pz - tr.sz[i], g);
+                    const double q = hash_charge(seed, i);
+                    for (int c = 0; c < kSampC; c++) a[c] = fma(g[c], q, a[c]);
                 }
             } else {
                 for (int l = 0; l < P; l++) {
                     const int lm = l % N1, lk = (l / N1) % N1, lj = l / (N1 * N1);
-                    a = fma(g_pot(px - proxy_node(tab.b[lj], sl[0], sh[0]), py - proxy_node(tab.b[lk], sl[1], sh[1]),
-                                  pz - proxy_node(tab.b[lm], sl[2], sh[2])),
-                            L.W[l], a);
+                    g_all(px - proxy_node(tab.b[lj], sl[0], sh[0]), py - proxy_node(tab.b[lk], sl[1], sh[1]),
+                          pz - proxy_node(tab.b[lm], sl[2], sh[2]), g);
+                    for (int c = 0; c < kSampC; c++) a[c] = fma(g[c], L.W[l], a[c]);
                 }
             }
-            L.Psi[k] = a;
+            for (int c = 0; c < kSampC; c++) L.Psi[c][k] = a[c];
         }
         __syncthreads();
-        double a = 0.0;
+        double a[kSampC] = {0.0, 0.0, 0.0, 0.0};
         if (t < nT) {
             for (int k = part; k < P; k += 4) {
                 const int m = k % N1, kk = (k / N1) % N1, j = k / (N1 * N1);
-                a = fma(L.LT[t][0][j] * L.LT[t][1][kk] * L.LT[t][2][m], L.Psi[k], a);
+                const double b = L.LT[t][0][j] * L.LT[t][1][kk] * L.LT[t][2][m];
+                for (int c = 0; c < kSampC; c++) a[c] = fma(b, L.Psi[c][k], a[c]);
             }
         }
-        L.red[part][t] = a;
+        for (int c = 0; c < kSampC; c++) L.red[part][c][t] = a[c];
         __syncthreads();
-        if (tid < nT) L.ap[kind][tid] = L.red[0][tid] + L.red[1][tid] + L.red[2][tid] + L.red[3][tid];
+        if (tid < nT)
+            for (int c = 0; c < kSampC; c++)
+                L.ap[kind][c][tid] = L.red[0][c][tid] + L.red[1][c][tid] + L.red[2][c][tid] + L.red[3][c][tid];
         __syncthreads();
     }
 }
@@ -181,12 +198,14 @@ __global__ void __launch_bounds__(TB) k_mac_samples(int np, const int2 *__restri
         L.tx[tid] = tr.sx[i];
         L.ty[tid] = tr.sy[i];
         L.tz[tid] = tr.sz[i];
-        double a = 0.0;  // the direct sum over the sampled sources
+        double a[kSampC] = {0.0, 0.0, 0.0, 0.0}, g[kSampC];  // the direct sum over the sampled sources
         for (int s = 0; s < nS; s++) {
             const int k = pick(tr.start[S], tr.p.count[S], nS, s);
-            a = fma(g_pot(L.tx[tid] - tr.sx[k], L.ty[tid] - tr.sy[k], L.tz[tid] - tr.sz[k]), hash_charge(seed, k), a);
+            g_all(L.tx[tid] - tr.sx[k], L.ty[tid] - tr.sy[k], L.tz[tid] - tr.sz[k], g);
+            const double q = hash_charge(seed, k);
+            for (int c = 0; c < kSampC; c++) a[c] = fma(g[c], q, a[c]);
         }
-        L.ex[tid] = a;
+        for (int c = 0; c < kSampC; c++) L.ex[c][tid] = a[c];
     }
     if (tid == 0) {
         float f[kMacIn];
@@ -203,11 +222,12 @@ __global__ void __launch_bounds__(TB) k_mac_samples(int np, const int2 *__restri
         }
         if (tid < 3) {
             double num = 0.0, den = 0.0;
-            for (int t = 0; t < nT; t++) {
-                const double d = L.ap[tid][t] - L.ex[t];
-                num += d * d;
-                den += L.ex[t] * L.ex[t];
-            }
+            for (int c = 0; c < kSampC; c++)
+                for (int t = 0; t < nT; t++) {
+                    const double d = L.ap[tid][c][t] - L.ex[c][t];
+                    num += d * d;
+                    den += L.ex[c][t] * L.ex[c][t];
+                }
             // log10 of the error, clamped to [-14, 0]: only errors near a
             // budget matter, and touching boxes give CC errors of 1e14
             const double e = den > 0 ? sqrt(num / den) : 1.0;

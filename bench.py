@@ -97,7 +97,8 @@ def parse():
 
 
 # per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (1e-5, 8e-6, 6e-6, 5e-6, 4.5e-6, 4e-6, 3.5e-6, 3.25e-6, 3e-6, 2.5e-6, 2e-6, 1e-6, 3e-7)
+MAC_BUDGETS = (2e-3, 1e-3, 6e-4, 4e-4, 3e-4, 2e-4, 1.5e-4, 1e-4, 8e-5, 6e-5, 5e-5, 4e-5, 3e-5, 2e-5, 1e-5, 5e-6,
+               2e-6)
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta):

@@ -268,8 +268,9 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
  * b2[64], W3[12][64], b3[12]) reads 10 features of a candidate cluster pair
  * (separation ratio (r_t + r_s)/d, r_t/d, r_s/d, log2 of both point counts,
  * both levels, log2(r_t/r_s), g/d and (r_t + r_s)/g with g the distance
- * between the boxes) and predicts log10 of the relative error of its PC, CP
- * and CC approximations at degrees 4, 6, 8 and 10 (output 4*type' + degree
+ * between the boxes) and predicts log10 of the relative error (L2 over the
+ * potential and its gradient) of its PC, CP and CC approximations at degrees
+ * 4, 6, 8 and 10 (output 4*type' + degree
  * index, type' = PC 0, CP 1, CC 2; scaled by 1/4, clamped to [-14, 0]).
  * With opts->mac = 1 (3D only; interp_degree 4, 6, 8 or 10, the highest
  * degree used) the traversal evaluates it on the non-PP pairs whose boxes are
@@ -294,8 +295,10 @@ int btk_solve_multi(const btk_run_config *cfg, const btk_options *opts, int m, c
  * n_samples pairs drawn from the pairs the traversal visits whose boxes are
  * apart with separation ratio in [0.3, 1.2), each with its features
  * d_X[n][10] and the measured errors
- * d_Y[n][12] of its approximations against the direct sum over up to 64
- * target and 1024 source points of the pair (charges U[-1,1] from seed).
+ * d_Y[n][12] of its approximations (relative L2 over the potential 1/r and
+ * the gradient kernel d/r^3, each component interpolated as the treecode
+ * does) against the direct sum over up to 64 target and 1024 source points
+ * of the pair (charges U[-1,1] from seed).
  * d_pairs[n][2] (may be null) receives each sample's (target, source) panel
  * ids.  *count receives the number of samples written (<= n_samples). */
 int btk_mac_sample(const btk_run_config *cfg, int n, const double *d_x, const double *d_y, const double *d_z,

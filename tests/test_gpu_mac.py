@@ -66,7 +66,8 @@ def _charge(seed, i):
 
 
 def _pair_errors(P, pts, start, T, S, seed, deg):
-    """PC, CP, CC relative errors of the pair at degree deg, restated on the CPU."""
+    """PC, CP, CC relative errors of the pair at degree deg (potential 1/r and
+    gradient kernel d/r^3, L2 over the four components), restated on the CPU."""
     def picks(p, m):
         c = P["point_count"][p]
         return [start[p] + (k * c) // m for k in range(m)]
@@ -76,7 +77,10 @@ def _pair_errors(P, pts, start, T, S, seed, deg):
     q = np.array([_charge(seed, i) for i in si])
     box = lambda p: (np.array([P["min_x"][p], P["min_y"][p], P["min_z"][p]]),
                      np.array([P["max_x"][p], P["max_y"][p], P["max_z"][p]]))
-    g = lambda a, b: 1.0 / np.linalg.norm(a[:, None, :] - b[None, :, :], axis=2)
+    def g(a, b):  # [4, len(a), len(b)]: 1/r and d/r^3
+        d = a[:, None, :] - b[None, :, :]
+        inv = 1.0 / np.linalg.norm(d, axis=2)
+        return np.concatenate([inv[None], np.moveaxis(d, 2, 0) * inv[None] ** 3])
     exact = g(t, s) @ q
     (tl, th), (sl, sh) = box(T), box(S)
 
@@ -92,8 +96,8 @@ def _pair_errors(P, pts, start, T, S, seed, deg):
     xs, xt = proxies(sl, sh), proxies(tl, th)
     Lt = basis(t, tl, th)
     pc = g(t, xs) @ W
-    cp = Lt @ (g(xt, s) @ q)
-    cc = Lt @ (g(xt, xs) @ W)
+    cp = np.einsum("tp,cp->ct", Lt, g(xt, s) @ q)   # every component interpolated on its own
+    cc = np.einsum("tp,cp->ct", Lt, g(xt, xs) @ W)
     return [np.linalg.norm(a - exact) / np.linalg.norm(exact) for a in (pc, cp, cc)]
 
 
