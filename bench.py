@@ -535,8 +535,11 @@ def main():
         "cpu_baseline": None,
     }
     # HBM bytes per launch of the near kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size)
-    tfile = os.path.join(HERE, "profiles", f"traffic_k_interact_near_{args.workload}_{head}.json")
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size) of the
+    # same kernel shape (mutual pairs or not, targets per lane)
+    sh = main_r.get("launch_shapes", {})
+    tfile = os.path.join(HERE, "profiles", f"traffic_k_interact_near_{args.workload}_{head}"
+                                           f"_m{sh.get('mutual', 0)}_t{sh.get('near_tpt', 0) or (2 if dim == 2 else 1)}.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
         line["roofline"] = dict(line["roofline"], traffic=t["traffic_bytes_per_launch"],
