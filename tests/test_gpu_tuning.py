@@ -153,3 +153,41 @@ def test_shared_matrix_cc_matches_pairwise(kernel, degree, learned):
     rel = float(torch.linalg.norm(outs["shared"] - outs["pairwise"]) / torch.linalg.norm(ref))
     print(f"  shared vs pairwise {rel:.2e}", flush=True)
     assert rel <= 5e-7
+
+
+@pytest.mark.parametrize("degree,split", [(10, False), (8, True)])
+def test_shared_matrix_cc_degree10_and_separate_trees(degree, split):
+    """The shared-matrix CC products at the largest degree (1331 proxies per
+    cluster), and with separate target and source trees (their own root
+    boxes: only centre offsets on the common lattice are keyed), against the
+    pairwise evaluation of the same list and the FP64 solve."""
+    print(f"\n[tuning] shared-matrix CC degree={degree} split={split}", flush=True)
+    n = 120000
+    d = _problem(3, n, 12)
+    if split:
+        xs, ys, zs, qs = wl.plummer_3d(90000, 13, a=0.3, rmax=3.0)
+        s = [torch.from_numpy(a).cuda() for a in (xs, ys, zs, qs)]
+    cfg = btk.make_config(3, degree, 300, 0.7, 0.0)
+    C = btk.components(3, btk.KERNEL_POT_GRAD)
+    outs, stats = {}, {}
+    for name, prec, cc in (("fp64", btk.PREC_FP64, 0), ("pairwise", btk.PREC_MIXED, -1),
+                           ("shared", btk.PREC_MIXED, 2)):
+        btk.set_tuning(cc_group=cc)
+        o = btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=int(not split), precision=prec)
+        out = torch.zeros(C * n, dtype=torch.float64, device="cuda")
+        if split:
+            stats[name] = btk.solve(cfg, o, d[0], d[1], d[2], s[3], out, xs=s[0], ys=s[1], zs=s[2])
+        else:
+            stats[name] = btk.solve(cfg, o, d[0], d[1], d[2], d[3], out)
+        outs[name] = out
+    print(f"  CC pairs {int(stats['shared'].count[3])}, shared {int(stats['shared'].cc_shared)}", flush=True)
+    if not split:
+        assert stats["shared"].cc_shared >= 0.3 * stats["shared"].count[3]
+    ref = outs["fp64"]
+    for name in ("pairwise", "shared"):
+        rel = float(torch.linalg.norm(outs[name] - ref) / torch.linalg.norm(ref))
+        print(f"  {name}: rel vs FP64 {rel:.2e}", flush=True)
+        assert rel <= 1e-6
+    rel = float(torch.linalg.norm(outs["shared"] - outs["pairwise"]) / torch.linalg.norm(ref))
+    print(f"  shared vs pairwise {rel:.2e}", flush=True)
+    assert rel <= 5e-7
