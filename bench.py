@@ -17,8 +17,8 @@ sampled targets above).  Launch shapes are autotuned before the warmup.
                     [--workload 2d|3d-uniform-100k|3d-sphere-plummer-4M|3d-plummer-16M|
                                 3d-batched-2M|3d-forest-256]
                     [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
-                    [--no-cpu-baseline | --cpu-baseline] [--no-accuracy] [--no-degree-scan]
-                    [--mac-budget B]
+                    [--no-cpu-baseline] [--cpu-stride K] [--no-accuracy] [--no-degree-scan]
+                    [--mac-budget B] [--n N] [--leaf L]
 """
 from __future__ import annotations
 
@@ -49,9 +49,10 @@ WORKLOADS = {
     "3d-sphere-plummer-4M": (3, 4_000_000, "sphere3", 8, 0.7, 2000, 2,
                              "3D potential+gradient, N=M=4,000,000 (half unit sphere, half Plummer a=0.3 r<=2), "
                              "q~U[-1,1], degree 8, theta 0.7, leaf 2000, eps 0, self excluded"),
-    "3d-plummer-16M": (3, 16_000_000, "plummer3", 8, 0.7, 1000, 2,
+    # leaf 1300: the fastest learned-MAC solve of the sizes 650 .. 1700 measured
+    "3d-plummer-16M": (3, 16_000_000, "plummer3", 8, 0.7, 1300, 2,
                        "3D potential+gradient, N=M=16,000,000 Plummer a=0.3 truncated at r<=10, q~U[-1,1], "
-                       "degree 8, theta 0.7 (the fixed MAC), leaf 1000, eps 0, self excluded"),
+                       "degree 8, theta 0.7 (the fixed MAC), leaf 1300, eps 0, self excluded"),
 }
 
 
@@ -81,16 +82,15 @@ def parse():
     ap.add_argument("--no-degree-scan", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
     ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--leaf", type=int, default=None, help="leaf size (cluster_size) instead of the workload's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-baseline", action="store_true",
-                    help="time the CPU oracle also on workloads above 1M points (minutes of CPU time)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--mac-budget", type=float, default=None,
                     help="learned MAC: this per-pair budget instead of the scan (for profiling)")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
-    ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline: every k-th PP/PC pair")
+    ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline up to 1M points: every k-th block of 64 PP/PC pairs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scale for the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -133,32 +133,49 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl_spec, x, y, z, q, stride):
+def cpu_strides(n, near_stride):
+    """Sampling strides (PP/PC pairs, CP/CC pairs, pass units) of the CPU
+    baseline: up to 1M points every near_stride-th PP/PC block only; above,
+    all four so that the sample stays near half a minute of host work."""
+    if n <= 1_000_000:
+        return near_stride, 1, 1
+    return max(1, n // 20_000), max(1, n // 20_000), max(1, n // 250_000)
+
+
+def cpu_baseline(wl_spec, x, y, z, q, near_stride, far_stride=1, pass_stride=1):
     """The CPU restatement (oracle) of the same workload, OpenMP on the host's
-    cores: tree, traversal, upward pass, CP/CC and downward pass over all
-    targets, PP/PC on every `stride`-th pair, scaled by the sampled fraction of
-    PP/PC evaluations."""
+    cores.  Every stride-th block of 64 consecutive units is run (PP/PC pairs,
+    CP/CC pairs, and the points, panels and leaves of the upward and downward
+    passes); each part's time is scaled by the fraction of its work it did
+    (evaluations for the pairs, units for the passes).  Tree and traversal run
+    in full."""
     from oracle import pyoracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(cores, 16))
     O.set_threads(cores)
     dim, _, _, deg, theta, leaf, kernel, _ = wl_spec
     cfg = O.make_config(dim, deg, leaf, theta, 0.0)
-    o = O.opts(kernel=kernel, convention=1, exclude_self=1, near_stride=stride)
+    o = O.opts(kernel=kernel, convention=1, exclude_self=1, near_stride=near_stride, far_stride=far_stride,
+               pass_stride=pass_stride)
     t0 = time.perf_counter()
     _, t, _ = O.solve(cfg, o, x, y, z, x, y, z, q)
     wall = time.perf_counter() - t0
-    near_s, frac = O.last_near()
-    est = wall - near_s + near_s / max(frac, 1e-12)
+    (near_s, near_f), (far_s, far_f) = O.last_near(), O.last_far()
+    pass_s, pass_f = O.last_passes()
+    parts = [(near_s, near_f), (far_s, far_f)] + list(zip(pass_s, pass_f))
+    est = wall + sum(s / max(f, 1e-12) - s for s, f in parts)
     return {
         "value": len(x) / est,
         "unit": "targets/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"oracle (C/OpenMP restatement) on the same {len(x)}-point problem: tree, traversal, upward, "
-                   f"CP/CC and downward over all targets; PP/PC on every {stride}th pair "
-                   f"({frac:.3f} of PP/PC evaluations, {near_s:.1f} s), scaled; measured {wall:.1f} s, "
+        "sample": (f"oracle (C/OpenMP restatement) on the same {len(x)}-point problem: tree and traversal in full; "
+                   f"every {near_stride}th block of 64 PP/PC pairs ({near_f:.4f} of their evaluations, "
+                   f"{near_s:.1f} s), every {far_stride}th of CP/CC ({far_f:.4f}, {far_s:.1f} s), every "
+                   f"{pass_stride}th block of 64 points/panels/leaves of the passes (fractions "
+                   f"{', '.join(f'{f:.4f}' for f in pass_f)}), each scaled; measured {wall:.1f} s, "
                    f"estimated full solve {est:.1f} s"),
+        "strides": {"near": near_stride, "far": far_stride, "passes": pass_stride},
     }
 
 
@@ -417,6 +434,10 @@ def main():
     spec = WORKLOADS[args.workload]
     dim, n, gen, degree, theta, leaf, kernel, desc = spec
     n = args.n or n
+    if args.leaf:
+        leaf = args.leaf
+        spec = spec[:5] + (leaf,) + spec[6:]
+        desc = f"{desc} [leaf {leaf}]"
     if gen == "gauss2":
         x, y, z, q = wl.gaussian_mixture_2d(n, SEED)
     elif gen == "cube3":
@@ -548,12 +569,12 @@ def main():
         line["mixed_over_plain"] = res["mixed"]["targets_per_s"] / res["plain"]["targets_per_s"]
     if "plain" in res and "learned" in res:
         line["learned_over_plain"] = res["learned"]["targets_per_s"] / res["plain"]["targets_per_s"]
-    # above 1M points the oracle's tree, traversal and far field alone take
-    # minutes, so the CPU leg runs there only when asked for
-    if not args.no_cpu_baseline and (n <= 1_000_000 or args.cpu_baseline):
+    if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
-        line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, args.cpu_stride)
+        line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, *cpu_strides(n, args.cpu_stride))
         line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        if "plain" in res:
+            line["plain_over_cpu"] = res["plain"]["targets_per_s"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
 

@@ -525,14 +525,47 @@ void orc_split_interactions(const orc_pair *list, int count, orc_pair *pp, orc_p
         *(ptr) += _v;          \
     } while (0)
 
-void orc_upward_pass(const orc_run_config *cfg, const orc_tree *ts, const double *x, const double *y,
-                     const double *z, const double *q, double *W) {
+/* Timing of the last orc_solve's passes (anterpolation, child-to-parent,
+ * parent-to-child, leaf-to-point): seconds and the fraction of each pass's
+ * units (points, panels, panels, targets) that it processed. */
+static double g_pass_seconds[4], g_pass_fraction[4] = {1, 1, 1, 1};
+void orc_last_passes(double *seconds, double *fraction) {
+    for (int i = 0; i < 4; i++) {
+        seconds[i] = g_pass_seconds[i];
+        fraction[i] = g_pass_fraction[i];
+    }
+}
+static double now_s(void);
+
+/* Sampling for CPU-baseline timing: unit i is processed when its block of 64
+ * consecutive units is every stride-th one, so that concurrent threads still
+ * meet on the same outputs as in the full loops (siblings, one target's pairs). */
+static inline int sampled(long long i, int stride) { return stride <= 1 || (i / 64) % stride == 0; }
+
+/* Fraction of the panels from lb on (non-leaves only if parents, weighted by
+ * child count; else all) that a pass with this stride processes. */
+static double sampled_fraction(const orc_tree *t, int lb, int stride, int parents) {
+    double all = 0, done = 0;
+    const int nc = t->dim == 3 ? 8 : 4;
+    for (int i = lb; i < t->level_start[t->levels]; i++) {
+        double w = parents ? (t->panels[i].is_leaf ? 0.0 : (double)nc) : 1.0;
+        all += w;
+        if (sampled(i, stride)) done += w;
+    }
+    return all > 0 ? done / all : 1.0;
+}
+
+/* stride > 1 (CPU-baseline sampling): only every stride-th point and panel */
+static void upward_pass(const orc_run_config *cfg, const orc_tree *ts, const double *x, const double *y,
+                        const double *z, const double *q, double *W, int stride) {
     const int dim = ts->dim, n = cfg->interp_degree, n1 = n + 1;
     const int P = ipow(n1, dim);
     const int N = ts->n_points;
+    double t0 = now_s();
     /* base_pots (upward_pass_impl.hpp:51-71, :133-159) */
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N; i++) {
+        if (!sampled(i, stride)) continue;
         int leaf = ts->point_leaf_panel[i];
         const orc_panel *L = &ts->panels[leaf];
         double bx[ORC_MAX_DEGREE + 1], by[ORC_MAX_DEGREE + 1], bz[ORC_MAX_DEGREE + 1];
@@ -557,11 +590,17 @@ void orc_upward_pass(const orc_run_config *cfg, const orc_tree *ts, const double
                 }
         }
     }
+    g_pass_seconds[0] = now_s() - t0;
+    long long nd = 0;
+    for (int i = 0; i < N; i++) nd += sampled(i, stride);
+    g_pass_fraction[0] = N > 0 ? (double)nd / N : 1.0;
+    t0 = now_s();
     /* child_to_parent per level, deepest to 1 (upward_pass.cpp:23-28) */
     for (int lev = ts->levels - 1; lev > 0; lev--) {
         int lb = ts->level_start[lev], ub = ts->level_start[lev + 1];
 #pragma omp parallel for schedule(dynamic)
         for (int i = lb; i < ub; i++) {
+            if (!sampled(i, stride)) continue;
             const orc_panel *C = &ts->panels[i];
             int parent = C->parent_id;
             if (parent <= -1) continue;
@@ -609,6 +648,13 @@ void orc_upward_pass(const orc_run_config *cfg, const orc_tree *ts, const double
             }
         }
     }
+    g_pass_seconds[1] = now_s() - t0;
+    g_pass_fraction[1] = sampled_fraction(ts, ts->levels > 1 ? ts->level_start[1] : 0, stride, 0);
+}
+
+void orc_upward_pass(const orc_run_config *cfg, const orc_tree *ts, const double *x, const double *y,
+                     const double *z, const double *q, double *W) {
+    upward_pass(cfg, ts, x, y, z, q, W, 1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -696,10 +742,11 @@ static inline void proxy_point(int dim, int n1, int idx, const double *cx, const
     }
 }
 
-static double g_near_seconds = 0, g_near_fraction = 1;
+static double g_near_seconds = 0, g_near_fraction = 1, g_far_seconds = 0, g_far_fraction = 1;
 double orc_last_near_seconds(void) { return g_near_seconds; }
 double orc_last_near_fraction(void) { return g_near_fraction; }
-static double now_s(void);
+double orc_last_far_seconds(void) { return g_far_seconds; }
+double orc_last_far_fraction(void) { return g_far_fraction; }
 
 void orc_fmm_interactions(const orc_run_config *cfg, const orc_opts *opt, const orc_tree *tt, const orc_tree *ts,
                           const double *xt, const double *yt, const double *zt, const double *xs,
@@ -712,28 +759,34 @@ void orc_fmm_interactions(const orc_run_config *cfg, const orc_opts *opt, const 
     const size_t pcT = (size_t)tt->panel_count;
     const kparams kp = make_kparams(dim, cfg->ker_eps, opt);
     static const double zero3[1] = {0};
-    const int stride = opt->near_stride > 1 ? opt->near_stride : 1;
-    double near_all = 0, near_done = 0;
+    /* sampling strides of the near (PP, PC) and far (CP, CC) pairs */
+    const int stride[2] = {opt->near_stride > 1 ? opt->near_stride : 1, opt->far_stride > 1 ? opt->far_stride : 1};
+    double all[2] = {0, 0}, done[2] = {0, 0};
     for (int i = 0; i < count; i++) {
-        int ty = list[i].interact_type;
-        if (ty > 1) continue;
+        int ty = list[i].interact_type, far = ty > 1;
         int it = list[i].target_panel, is = list[i].source_panel;
-        double e = (double)(tt->leaf_start[it + 1] - tt->leaf_start[it]) *
-                   (ty == 0 ? (double)(ts->leaf_start[is + 1] - ts->leaf_start[is]) : (double)P);
-        near_all += e;
-        if (i % stride == 0) near_done += e;
+        double nt = (double)(tt->leaf_start[it + 1] - tt->leaf_start[it]);
+        double ns = (double)(ts->leaf_start[is + 1] - ts->leaf_start[is]);
+        double e = (ty == 0 || ty == 1 ? nt : (double)P) * (ty == 0 || ty == 2 ? ns : (double)P);
+        all[far] += e;
+        if (sampled(i, stride[far])) done[far] += e;
     }
-    g_near_fraction = near_all > 0 ? near_done / near_all : 1.0;
+    g_near_fraction = all[0] > 0 ? done[0] / all[0] : 1.0;
+    g_far_fraction = all[1] > 0 ? done[1] / all[1] : 1.0;
     double tn0 = 0;
 
     /* one launch per type, PP, PC, CP, CC (poisson_fmm.cpp:21-24) */
     for (int type = 0; type < 4; type++) {
         if (type == 0) tn0 = now_s();
-        if (type == 2) g_near_seconds = now_s() - tn0;
+        if (type == 2) {
+            g_near_seconds = now_s() - tn0;
+            tn0 = now_s();
+        }
+        const int st = stride[type > 1];
 #pragma omp parallel for schedule(dynamic)
         for (int i = 0; i < count; i++) {
             if (list[i].interact_type != type) continue;
-            if (type < 2 && i % stride != 0) continue;
+            if (!sampled(i, st)) continue;
             const int it = list[i].target_panel, is = list[i].source_panel;
             const orc_panel *T = &tt->panels[it], *S = &ts->panels[is];
             double tcx[ORC_MAX_DEGREE + 1], tcy[ORC_MAX_DEGREE + 1], tcz[ORC_MAX_DEGREE + 1];
@@ -793,6 +846,7 @@ void orc_fmm_interactions(const orc_run_config *cfg, const orc_opts *opt, const 
             }
         }
     }
+    g_far_seconds = now_s() - tn0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -807,6 +861,9 @@ void orc_downward_pass(const orc_run_config *cfg, const orc_opts *opt, const orc
     const int C = orc_components(dim, opt->kernel);
     const int M = tt->n_points;
     const size_t pc = (size_t)tt->panel_count;
+    /* >1 (CPU-baseline sampling): only every stride-th panel and leaf */
+    const int stride = opt->pass_stride > 1 ? opt->pass_stride : 1;
+    double t0 = now_s();
     /* parent_to_child, levels 0..L-2 (downward_pass.cpp:21-25) */
     for (int lev = 0; lev < tt->levels - 1; lev++) {
         int lb = tt->level_start[lev], ub = tt->level_start[lev + 1];
@@ -815,6 +872,7 @@ void orc_downward_pass(const orc_run_config *cfg, const orc_opts *opt, const orc
             for (int cj = 0; cj < nc; cj++) {
                 const orc_panel *Pp = &tt->panels[i];
                 if (Pp->is_leaf) continue; /* reference reads child -1 here (undefined) */
+                if (!sampled(i, stride)) continue;
                 int child = Pp->child[cj];
                 const orc_panel *Cc = &tt->panels[child];
                 double cx[ORC_MAX_DEGREE + 1], cy[ORC_MAX_DEGREE + 1], cz[ORC_MAX_DEGREE + 1];
@@ -865,12 +923,23 @@ void orc_downward_pass(const orc_run_config *cfg, const orc_opts *opt, const orc
     }
     /* leaf_to_point (downward_pass.cpp:26-28): deepest level only in the
      * reference; every leaf in the corrected form */
+    g_pass_seconds[2] = now_s() - t0;
+    g_pass_fraction[2] = sampled_fraction(tt, 0, stride, 1);
+    t0 = now_s();
     int lb = opt->deepest_only ? tt->level_start[tt->levels - 1] : 0;
     int ub = tt->level_start[tt->levels];
+    double all = 0, done = 0;
+    for (int i = lb; i < ub; i++) {
+        if (!tt->panels[i].is_leaf) continue;
+        double np = (double)(tt->leaf_start[i + 1] - tt->leaf_start[i]);
+        all += np;
+        if (sampled(i, stride)) done += np;
+    }
 #pragma omp parallel for schedule(dynamic)
     for (int i = lb; i < ub; i++) {
         const orc_panel *L = &tt->panels[i];
         if (!L->is_leaf) continue;
+        if (!sampled(i, stride)) continue;
         for (int a = tt->leaf_start[i]; a < tt->leaf_start[i + 1]; a++) {
             int i_t = tt->leaf_points[a];
             double bx[ORC_MAX_DEGREE + 1], by[ORC_MAX_DEGREE + 1], bz[ORC_MAX_DEGREE + 1];
@@ -898,6 +967,8 @@ void orc_downward_pass(const orc_run_config *cfg, const orc_opts *opt, const orc
             }
         }
     }
+    g_pass_seconds[3] = now_s() - t0;
+    g_pass_fraction[3] = all > 0 ? done / all : 1.0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -951,7 +1022,7 @@ int orc_solve(const orc_run_config *cfg_in, const orc_opts *opt, int m, const do
     const int P = ipow(cfg.interp_degree + 1, dim);
     double *W = (double *)calloc((size_t)ts.panel_count * P, sizeof(double));
     double *Psi = (double *)calloc((size_t)C * T->panel_count * P, sizeof(double));
-    orc_upward_pass(&cfg, &ts, xs, ys, zs, q, W);
+    upward_pass(&cfg, &ts, xs, ys, zs, q, W, opt->pass_stride > 1 ? opt->pass_stride : 1);
     double t3 = now_s();
     orc_fmm_interactions(&cfg, opt, T, &ts, xt, yt, zt, xs, ys, zs, q, W, list, count, out, Psi);
     double t4 = now_s();

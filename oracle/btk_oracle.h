@@ -105,13 +105,23 @@ typedef struct orc_opts {
     int convention;      /* 0 = reference signs (poisson_fmm_impl.hpp:38,241); 1 = physical (negated) */
     int exclude_self;    /* 1 = skip pairs at zero distance (dx=dy=dz=0) */
     int deepest_only;    /* 1 = leaf_to_point only on the deepest level (downward_pass.cpp:14-16) */
-    int near_stride;     /* >1: evaluate only every near_stride-th PP/PC pair (CPU-baseline sampling) */
+    /* CPU-baseline sampling, >1: only every stride-th block of 64 consecutive
+     * units is processed (list pairs; points, panels and leaves of the passes) */
+    int near_stride;     /* PP/PC pairs */
+    int far_stride;      /* CP/CC pairs */
+    int pass_stride;     /* upward (orc_solve) and downward pass */
 } orc_opts;
 
 /* Seconds spent in the PP/PC loops of the last orc_fmm_interactions call, and
- * the fraction of PP/PC kernel evaluations that call performed. */
+ * the fraction of PP/PC kernel evaluations that call performed; the same for
+ * the CP/CC loops. */
 double orc_last_near_seconds(void);
 double orc_last_near_fraction(void);
+double orc_last_far_seconds(void);
+double orc_last_far_fraction(void);
+/* Seconds and processed fraction of the last orc_solve's anterpolation,
+ * child-to-parent, parent-to-child and leaf-to-point passes (4 each). */
+void orc_last_passes(double *seconds, double *fraction);
 
 int orc_components(int dim, int kernel);
 

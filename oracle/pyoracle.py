@@ -52,7 +52,8 @@ class Pair(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
-                ("deepest_only", C.c_int), ("near_stride", C.c_int)]
+                ("deepest_only", C.c_int), ("near_stride", C.c_int), ("far_stride", C.c_int),
+                ("pass_stride", C.c_int)]
 
 
 PAIR_DTYPE = np.dtype([("target_panel", "<i4"), ("source_panel", "<i4"), ("interact_type", "<i4")])
@@ -114,6 +115,9 @@ def lib():
         L.orc_components.argtypes = [C.c_int, C.c_int]
         L.orc_last_near_seconds.restype = C.c_double
         L.orc_last_near_fraction.restype = C.c_double
+        L.orc_last_far_seconds.restype = C.c_double
+        L.orc_last_far_fraction.restype = C.c_double
+        L.orc_last_passes.argtypes = [dp, dp]
         L.free = C.CDLL(None).free
         L.free.argtypes = [C.c_void_p]
         _lib = L
@@ -231,16 +235,30 @@ def split_interactions(lst: np.ndarray, cfg: RunConfig):
     return pp, pc, cp, cc
 
 
-def opts(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, near_stride=1) -> Opts:
+def opts(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, near_stride=1, far_stride=1,
+         pass_stride=1) -> Opts:
     o = Opts()
     o.kernel, o.convention, o.exclude_self, o.deepest_only = kernel, convention, exclude_self, deepest_only
-    o.near_stride = near_stride
+    o.near_stride, o.far_stride, o.pass_stride = near_stride, far_stride, pass_stride
     return o
 
 
 def last_near():
     """(seconds in the PP/PC loops, fraction of PP/PC evaluations performed) of the last call."""
     return lib().orc_last_near_seconds(), lib().orc_last_near_fraction()
+
+
+def last_passes():
+    """(seconds, processed fraction) of the last solve's anterpolation,
+    child-to-parent, parent-to-child and leaf-to-point passes."""
+    sec, frac = np.zeros(4), np.zeros(4)
+    lib().orc_last_passes(_dp(sec), _dp(frac))
+    return sec, frac
+
+
+def last_far():
+    """(seconds in the CP/CC loops, fraction of CP/CC evaluations performed) of the last call."""
+    return lib().orc_last_far_seconds(), lib().orc_last_far_fraction()
 
 
 def components(dim, kernel):

@@ -245,3 +245,20 @@ def test_degree_change_identities(d):
     A_bad = _degree_matrix(d, n)[::-1]
     bad = np.einsum("ak,bl,cm,pabc->pklm", A_bad, A, A, Wn).reshape(Wd.shape)
     assert np.abs(bad - Wd).max() > 1e-6 * np.abs(Wd).max()
+
+
+def test_sampled_solve_reports_its_fractions():
+    """The CPU baseline's sampling: strides of 1 run everything (fractions 1);
+    larger strides run about one block of 64 units in `stride` of every part
+    and report the fraction of each part's work they did."""
+    x, y, z, q = wl.uniform_cube(15_000, 3, 5)
+    cfg = O.make_config(3, 2, 8, 0.7, 0.0)
+    O.solve(cfg, O.opts(kernel=O.KERNEL_POT_GRAD), x, y, z, x, y, z, q)
+    assert O.last_near()[1] == 1.0 and O.last_far()[1] == 1.0
+    assert np.all(O.last_passes()[1] == 1.0)
+    O.solve(cfg, O.opts(kernel=O.KERNEL_POT_GRAD, near_stride=4, far_stride=5, pass_stride=3), x, y, z, x, y, z, q)
+    assert abs(O.last_near()[1] - 1 / 4) < 0.08
+    assert abs(O.last_far()[1] - 1 / 5) < 0.08
+    sec, frac = O.last_passes()
+    assert np.all(np.abs(frac - 1 / 3) < 0.12), frac
+    assert np.all(sec >= 0)
