@@ -92,6 +92,35 @@ __device__ __forceinline__ void split_f32(double x, float &hi, float &lo) {
     lo = (float)(x - (double)hi);
 }
 
+__device__ __forceinline__ double sel3(int d, double a, double b, double c) { return d == 0 ? a : (d == 1 ? b : c); }
+
+// Near pairs of separated boxes take the far field's relative coordinates
+// instead of hi/lo pairs (one FP32 subtraction per axis instead of three).
+// Relative to the target box centre c, |t - c| + |s - c| <= r_t + d + r_s
+// (d the distance of the centres), while |t - s| >= g, the gap between the
+// boxes; rounding t - c and s - c to FP32 then perturbs t - s by at most
+// (r_t + d + r_s) / g times 2^-24, relative.  The pair qualifies when that
+// factor is at most (1 + 0.7) / (1 - 0.7), the far field's own bound at
+// theta = 0.7 (3.4e-7).
+constexpr double kRelFactor = 17.0 / 3.0;
+template <int DIM>
+__device__ __forceinline__ bool separated_for_rel(int T, int S, const TgtView &tv, const SrcView &sv) {
+    double g2 = 0.0, d2 = 0.0, et2 = 0.0, es2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+        const double tl = sel3(a, tv.lo0[T], tv.lo1[T], tv.lo2[T]), th = sel3(a, tv.hi0[T], tv.hi1[T], tv.hi2[T]);
+        const double sl = sel3(a, sv.lo0[S], sv.lo1[S], sv.lo2[S]), sh = sel3(a, sv.hi0[S], sv.hi1[S], sv.hi2[S]);
+        const double g = fmax(0.0, fmax(sl - th, tl - sh));
+        const double dc = 0.5 * ((sl + sh) - (tl + th));
+        g2 += g * g;
+        d2 += dc * dc;
+        et2 += (th - tl) * (th - tl);
+        es2 += (sh - sl) * (sh - sl);
+    }
+    const double g = sqrt(g2);
+    return g > 0.0 && 0.5 * sqrt(et2) + 0.5 * sqrt(es2) + sqrt(d2) <= kRelFactor * g;
+}
+
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
 // sorted point of its points (0 for proxy sources), z = number of sources
 // (the leaf's point count, or P proxy points), w = interact_type.
@@ -99,7 +128,6 @@ struct EntryView {
     const int4 *meta;
 };
 
-__device__ __forceinline__ double sel3(int d, double a, double b, double c) { return d == 0 ? a : (d == 1 ? b : c); }
 
 // The proxy point idx of a box: nodes cos(pi/n*j) mapped to each axis
 // (bli_points_shift, bli_impl.hpp:65-77), flat index (j*(n+1)+k)*(n+1)+l.
@@ -227,7 +255,7 @@ __global__ void __launch_bounds__(kLanes)
     float tr[TPT][3], th[TPT][3], tl[TPT][3];
     double acc[TPT][C];
     double ctr[3] = {0.0, 0.0, 0.0};
-    if constexpr (REL) {
+    if constexpr (REL || SPLIT) {
         ctr[0] = 0.5 * (tv.lo0[T] + tv.hi0[T]);
         ctr[1] = 0.5 * (tv.lo1[T] + tv.hi1[T]);
         if constexpr (DIM == 3) ctr[2] = 0.5 * (tv.lo2[T] + tv.hi2[T]);
@@ -252,21 +280,38 @@ __global__ void __launch_bounds__(kLanes)
 #pragma unroll
         for (int c = 0; c < C; c++) acc[k][c] = 0.0;
     }
-    auto fetch = [&](const int4 &mm, int j) -> Tile {
-        if constexpr (REL)
-            return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
-        else if constexpr (SPLIT)
-            return load_source_split<DIM>(mm, j, n1, tabs, sv);
+    // near 3D mixed: entries of separated boxes fill the hi/lo tile with
+    // relative coordinates (hi fields) instead
+    auto rel_entry = [&](const int4 &mm) {
+        if constexpr (SPLIT)
+            return separated_for_rel<DIM>(T, mm.x, tv, sv);
         else
-            return load_source<DIM, F32>(mm, j, n1, tabs, sv);
+            return false;
     };
-    auto eval = [&](const Tile &s, int k, float *a32, auto excl) {
+    auto fetch = [&](const int4 &mm, int j, bool rel) -> Tile {
+        if constexpr (REL) {
+            return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
+        } else if constexpr (SPLIT) {
+            if (rel) {
+                const TileRel r = load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
+                return TileSplit{r.x, r.y, r.z, r.q, 0.0f, 0.0f, 0.0f, 0.0f};
+            }
+            return load_source_split<DIM>(mm, j, n1, tabs, sv);
+        } else {
+            return load_source<DIM, F32>(mm, j, n1, tabs, sv);
+        }
+    };
+    auto eval = [&](const Tile &s, int k, float *a32, auto excl, auto relc) {
         constexpr bool X = decltype(excl)::value;
         if constexpr (REL)
             eval_pair_rel<DIM, KERNEL, X>(s, tr[k], (float)kp.eps2, a32);
-        else if constexpr (SPLIT)
-            eval_pair_split<DIM, KERNEL, X>(s, th[k], tl[k], (float)kp.eps2, a32);
-        else
+        else if constexpr (SPLIT) {
+            if constexpr (decltype(relc)::value)
+                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.xh, tr[k][1] - s.yh, tr[k][2] - s.zh, s.q, (float)kp.eps2,
+                                         a32);
+            else
+                eval_pair_split<DIM, KERNEL, X>(s, th[k], tl[k], (float)kp.eps2, a32);
+        } else
             eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
     };
     // The entries' sources stream through the tile 64 at a time, as one
@@ -276,49 +321,66 @@ __global__ void __launch_bounds__(kLanes)
     int4 m = make_int4(0, 0, 0, 0);
     while (e < e1 && (m = ev.meta[e]).z == 0) e++;
     Tile next{};
-    if (e < e1) next = fetch(m, lane);
+    bool rel = false;
+    if (e < e1) {
+        rel = rel_entry(m);
+        next = fetch(m, lane, rel);
+    }
     while (e < e1) {
         tile[lane] = next;
         __syncthreads();
         const int nt = min(kLanes, m.z - j0);
         // With one tree for targets and sources, coincident points share a
         // leaf, so among point sources only the target leaf's own entry can
-        // hold a zero-distance pair.  Proxy sources keep the test.
-        const bool check = !same_tree || m.x == T || (m.w & 1);
+        // hold a zero-distance pair.  Proxy sources keep the test, except
+        // those of separated boxes.
+        const bool check = (!same_tree || m.x == T || (m.w & 1)) && !rel;
         int en = e, jn = j0 + kLanes;
         int4 mn = m;
+        bool reln = rel;
         if (jn >= m.z) {
             jn = 0;
             en++;
             while (en < e1 && (mn = ev.meta[en]).z == 0) en++;
+            if (en < e1) reln = rel_entry(mn);
         }
-        if (en < e1) next = fetch(mn, jn + lane);
+        if (en < e1) next = fetch(mn, jn + lane, reln);
         float a32[TPT][C];
 #pragma unroll
         for (int k = 0; k < TPT; k++)
 #pragma unroll
             for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
-        if (nt == kLanes && EXCL && !check) {
-            // no zero-distance pair is possible here: skip the exclusion test
+        auto run_tile = [&](auto relc) {
+            if (nt == kLanes && EXCL && !check) {
+                // no zero-distance pair is possible here: skip the exclusion test
 #pragma unroll 8
-            for (int jj = 0; jj < kLanes; jj++) {
-                const Tile s = tile[jj];
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Tile s = tile[jj];
 #pragma unroll
-                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{});
-            }
-        } else if (nt == kLanes) {
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{}, relc);
+                }
+            } else if (nt == kLanes) {
 #pragma unroll 8
-            for (int jj = 0; jj < kLanes; jj++) {
-                const Tile s = tile[jj];
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Tile s = tile[jj];
 #pragma unroll
-                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{});
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, relc);
+                }
+            } else {
+                for (int jj = 0; jj < nt; jj++) {
+                    const Tile s = tile[jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, relc);
+                }
             }
+        };
+        if constexpr (SPLIT) {
+            if (rel)
+                run_tile(std::true_type{});
+            else
+                run_tile(std::false_type{});
         } else {
-            for (int jj = 0; jj < nt; jj++) {
-                const Tile s = tile[jj];
-#pragma unroll
-                for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{});
-            }
+            run_tile(std::false_type{});
         }
         if constexpr (F32) {
 #pragma unroll
@@ -330,6 +392,7 @@ __global__ void __launch_bounds__(kLanes)
         e = en;
         j0 = jn;
         m = mn;
+        rel = reln;
     }
 #pragma unroll
     for (int k = 0; k < TPT; k++) {
