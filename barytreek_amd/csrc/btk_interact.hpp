@@ -234,8 +234,11 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
     return (r / kRun) * (kXcds * kRun) + x * kRun + (r % kRun);
 }
 
+// The mixed-precision kernels ask for at least 6 wavefronts per SIMD (at most
+// 80 VGPRs); left to itself the compiler spends up to 113 on unrolling and
+// runs 4.  The FP64 kernels keep the compiler's choice.
 template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL>
-__global__ void __launch_bounds__(kLanes)
+__global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
@@ -380,7 +383,28 @@ __global__ void __launch_bounds__(kLanes)
             else
                 run_tile(std::false_type{});
         } else {
-            run_tile(std::false_type{});
+            if (nt == kLanes && EXCL && !check) {
+                // no zero-distance pair is possible here: skip the exclusion test
+#pragma unroll 8
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Tile s = tile[jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{}, std::false_type{});
+                }
+            } else if (nt == kLanes) {
+#pragma unroll 8
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Tile s = tile[jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, std::false_type{});
+                }
+            } else {
+                for (int jj = 0; jj < nt; jj++) {
+                    const Tile s = tile[jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, std::false_type{});
+                }
+            }
         }
         if constexpr (F32) {
 #pragma unroll
