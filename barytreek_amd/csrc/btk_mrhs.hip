@@ -61,8 +61,10 @@ struct MView {
     int K, kp;
 };
 
+// Occupancy asked of the compiler: 5 wavefronts per SIMD for one component
+// (96 VGPRs, no spills; 4 by its own choice), 2 for three or four (1).
 template <int DIM, int KERNEL, bool FAR, bool EXCL>
-__global__ void __launch_bounds__(TB)
+__global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2))
     k_interact_mrhs(TaskView tk, GroupView gv, TgtView tv, SrcView sv, MView mv, int n1, NodeTables tabs,
                     KParams kp, int M, int pct, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
@@ -373,8 +375,10 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
 // FP32 tile products are added into FP64 sums; each accumulator row is one
 // pair, its lanes 32 consecutive target proxies, so the FP64 atomic adds
 // into each pair's own target cluster are coalesced.
+// At least 3 wavefronts per SIMD (155 VGPRs, no spills; 2 by the compiler's
+// own choice).
 template <int KERNEL, int C0, int NC, bool EXCL>
-__global__ void __launch_bounds__(TB)
+__global__ void __launch_bounds__(TB, 3)
     k_cc_gemm(const int4 *__restrict__ items, const int *__restrict__ entT, const int *__restrict__ entS,
               LevelBoxes TB_, LevelBoxes SB, const double *__restrict__ W, int n1, NodeTables tabs, KParams kp,
               double *__restrict__ out, size_t comp_stride) {
