@@ -18,7 +18,7 @@ sampled targets above).  Launch shapes are autotuned before the warmup.
                                 3d-batched-2M|3d-forest-256]
                     [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
                     [--no-cpu-baseline] [--cpu-stride K] [--no-accuracy] [--no-degree-scan]
-                    [--mac-budget B] [--n N] [--leaf L]
+                    [--mac-budget B] [--n N] [--leaf L] [--degree D]
 """
 from __future__ import annotations
 
@@ -83,6 +83,7 @@ def parse():
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--leaf", type=int, default=None, help="leaf size (cluster_size) instead of the workload's")
+    ap.add_argument("--degree", type=int, default=None, help="interpolation degree instead of the workload's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -438,6 +439,10 @@ def main():
         leaf = args.leaf
         spec = spec[:5] + (leaf,) + spec[6:]
         desc = f"{desc} [leaf {leaf}]"
+    if args.degree:
+        degree = args.degree
+        spec = spec[:3] + (degree,) + spec[4:]
+        desc = f"{desc} [degree {degree}]"
     if gen == "gauss2":
         x, y, z, q = wl.gaussian_mixture_2d(n, SEED)
     elif gen == "cube3":
