@@ -84,6 +84,9 @@ def parse():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--leaf", type=int, default=None, help="leaf size (cluster_size) instead of the workload's")
     ap.add_argument("--degree", type=int, default=None, help="interpolation degree instead of the workload's")
+    ap.add_argument("--mac-sets", type=int, default=10, help="learned MAC: Plummer sets sampled for training")
+    ap.add_argument("--mac-pairs", type=int, default=100_000, help="learned MAC: cluster pairs sampled per set")
+    ap.add_argument("--mac-epochs", type=int, default=20, help="learned MAC: training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -98,20 +101,19 @@ def parse():
 
 
 # per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (2e-3, 1e-3, 6e-4, 4e-4, 3e-4, 2e-4, 1.5e-4, 1e-4, 8e-5, 6e-5, 5e-5, 4e-5, 3e-5, 2e-5, 1e-5, 5e-6,
-               2e-6)
+MAC_BUDGETS = (2e-3, 1e-3, 6e-4, 4e-4, 3e-4, 2e-4, 1.5e-4, 1e-4, 8e-5, 7e-5, 6e-5, 5.5e-5, 5e-5, 4.5e-5, 4e-5,
+               3.5e-5, 3e-5, 2e-5, 1e-5, 5e-6, 2e-6)
 
 
-def train_mac(btk, torch, wl, degree, leaf, theta):
-    """Learned MAC for 3D Plummer-like sets: 1M cluster pairs sampled from the
-    treecode's own runs on ten 2M-point Plummer sets (a = 0.3, other seeds
-    than the workload's), 100000 pairs each, then 20 epochs of Adam on
+def train_mac(btk, torch, wl, degree, leaf, theta, sets=10, per=100_000, epochs=20):
+    """Learned MAC for 3D Plummer-like sets: cluster pairs sampled from the
+    treecode's own runs on `sets` 2M-point Plummer sets (a = 0.3, other seeds
+    than the workload's), `per` pairs each, then `epochs` epochs of Adam on
     minibatches of 4096; returns the training record."""
     cfg = btk.make_config(3, degree, leaf, theta, 0.0)
     Xs, Ys, total = [], [], 0
     t0 = time.perf_counter()
-    per = 100_000
-    for seed in range(11, 21):
+    for seed in range(11, 11 + sets):
         x, y, z, _ = wl.plummer_3d(2_000_000, seed, a=0.3, rmax=10.0)
         d = [torch.from_numpy(a).cuda() for a in (x, y, z)]
         X = torch.zeros(per * 10, dtype=torch.float32, device="cuda")
@@ -123,10 +125,12 @@ def train_mac(btk, torch, wl, degree, leaf, theta):
         del d
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    loss = btk.mac_train(torch.cat(Xs), torch.cat(Ys), total, epochs=20, batch=4096, lr=2e-3, seed=3, reinit=True)
+    loss = btk.mac_train(torch.cat(Xs), torch.cat(Ys), total, epochs=epochs, batch=4096, lr=2e-3, seed=3,
+                         reinit=True)
     torch.cuda.synchronize()
     return {"samples": total, "sample_s": t1 - t0, "train_s": time.perf_counter() - t1, "loss": loss,
-            "sets": "10 Plummer sets of 2M points (a=0.3, seeds 11-20), 100000 pairs each"}
+            "epochs": epochs,
+            "sets": f"{sets} Plummer sets of 2M points (a=0.3, seeds 11-{10 + sets}), {per} pairs each"}
 
 
 def log(msg):
@@ -482,7 +486,7 @@ def main():
         train = None
         if v == "learned":  # untimed, like the warmup
             log("learned MAC: samples and training")
-            train = train_mac(btk, torch, wl, degree, leaf, theta)
+            train = train_mac(btk, torch, wl, degree, leaf, theta, args.mac_sets, args.mac_pairs, args.mac_epochs)
         btk.set_tuning()  # defaults
         if args.shapes:
             btk.set_tuning(*[int(x) for x in args.shapes.split(",")])
