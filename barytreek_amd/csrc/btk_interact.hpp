@@ -105,21 +105,31 @@ __device__ __forceinline__ double sel3(int d, double a, double b, double c) { re
 constexpr double kRelFactor = 17.0 / 3.0;
 template <int DIM>
 __device__ __forceinline__ bool separated_for_rel(int T, int S, const TgtView &tv, const SrcView &sv) {
+    const double tl[3] = {tv.lo0[T], tv.lo1[T], DIM == 3 ? tv.lo2[T] : 0.0};
+    const double th[3] = {tv.hi0[T], tv.hi1[T], DIM == 3 ? tv.hi2[T] : 0.0};
+    const double sl[3] = {sv.lo0[S], sv.lo1[S], DIM == 3 ? sv.lo2[S] : 0.0};
+    const double sh[3] = {sv.hi0[S], sv.hi1[S], DIM == 3 ? sv.hi2[S] : 0.0};
     double g2 = 0.0, d2 = 0.0, et2 = 0.0, es2 = 0.0;
 #pragma unroll
     for (int a = 0; a < DIM; a++) {
-        const double tl = sel3(a, tv.lo0[T], tv.lo1[T], tv.lo2[T]), th = sel3(a, tv.hi0[T], tv.hi1[T], tv.hi2[T]);
-        const double sl = sel3(a, sv.lo0[S], sv.lo1[S], sv.lo2[S]), sh = sel3(a, sv.hi0[S], sv.hi1[S], sv.hi2[S]);
-        const double g = fmax(0.0, fmax(sl - th, tl - sh));
-        const double dc = 0.5 * ((sl + sh) - (tl + th));
+        const double g = fmax(0.0, fmax(sl[a] - th[a], tl[a] - sh[a]));
+        const double dc = 0.5 * ((sl[a] + sh[a]) - (tl[a] + th[a]));
         g2 += g * g;
         d2 += dc * dc;
-        et2 += (th - tl) * (th - tl);
-        es2 += (sh - sl) * (sh - sl);
+        et2 += (th[a] - tl[a]) * (th[a] - tl[a]);
+        es2 += (sh[a] - sl[a]) * (sh[a] - sl[a]);
     }
     const double g = sqrt(g2);
     return g > 0.0 && 0.5 * sqrt(et2) + 0.5 * sqrt(es2) + sqrt(d2) <= kRelFactor * g;
 }
+
+// A source of the 2D near kernels in mixed precision: FP64 coordinates
+// (t - s formed in FP64 and rounded once), or, for entries of separated
+// boxes, FP32 coordinates relative to the target box centre (rx, ry).
+struct TileMix2 {
+    double x, y;
+    float rx, ry, q, pad;
+};
 
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
 // sorted point of its points (0 for proxy sources), z = number of sources
@@ -244,7 +254,9 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
     constexpr bool SPLIT = F32 && !FAR && DIM == 3;  // FP32 hi/lo coordinates (TileSplit)
-    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<SPLIT, TileSplit, TileSrc<F32>>>;
+    constexpr bool MIX2 = F32 && !FAR && DIM == 2;   // FP64 or relative FP32 coordinates (TileMix2)
+    using Tile = std::conditional_t<
+        REL, TileRel, std::conditional_t<SPLIT, TileSplit, std::conditional_t<MIX2, TileMix2, TileSrc<F32>>>>;
     __shared__ Tile tile[kLanes];
     const int lane = threadIdx.x;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -258,7 +270,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     float tr[TPT][3], th[TPT][3], tl[TPT][3];
     double acc[TPT][C];
     double ctr[3] = {0.0, 0.0, 0.0};
-    if constexpr (REL || SPLIT) {
+    if constexpr (REL || SPLIT || MIX2) {
         ctr[0] = 0.5 * (tv.lo0[T] + tv.hi0[T]);
         ctr[1] = 0.5 * (tv.lo1[T] + tv.hi1[T]);
         if constexpr (DIM == 3) ctr[2] = 0.5 * (tv.lo2[T] + tv.hi2[T]);
@@ -286,7 +298,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     // near 3D mixed: entries of separated boxes fill the hi/lo tile with
     // relative coordinates (hi fields) instead
     auto rel_entry = [&](const int4 &mm) {
-        if constexpr (SPLIT)
+        if constexpr (SPLIT || MIX2)
             return separated_for_rel<DIM>(T, mm.x, tv, sv);
         else
             return false;
@@ -300,6 +312,10 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
                 return TileSplit{r.x, r.y, r.z, r.q, 0.0f, 0.0f, 0.0f, 0.0f};
             }
             return load_source_split<DIM>(mm, j, n1, tabs, sv);
+        } else if constexpr (MIX2) {
+            const TileSrc<true> a = load_source<DIM, true>(mm, j, n1, tabs, sv);
+            if (rel) return TileMix2{0.0, 0.0, (float)(a.x - ctr[0]), (float)(a.y - ctr[1]), a.q, 0.0f};
+            return TileMix2{a.x, a.y, 0.0f, 0.0f, a.q, 0.0f};
         } else {
             return load_source<DIM, F32>(mm, j, n1, tabs, sv);
         }
@@ -314,6 +330,12 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
                                          a32);
             else
                 eval_pair_split<DIM, KERNEL, X>(s, th[k], tl[k], (float)kp.eps2, a32);
+        } else if constexpr (MIX2) {
+            if constexpr (decltype(relc)::value)
+                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.rx, tr[k][1] - s.ry, 0.0f, s.q, (float)kp.eps2, a32);
+            else
+                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f, s.q,
+                                         (float)kp.eps2, a32);
         } else
             eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
     };
@@ -377,7 +399,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
                 }
             }
         };
-        if constexpr (SPLIT) {
+        if constexpr (SPLIT || MIX2) {
             if (rel)
                 run_tile(std::true_type{});
             else
