@@ -1,5 +1,6 @@
 // The C ABI of include/btk.h.  Every entry point converts exceptions into a
 // status code and a thread-local message.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -594,39 +595,44 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
         const btk_tuning saved = btk::tuning();
         btk::KernelEvents kev;
         for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
-        float best_near = 1e30f, best_far = 1e30f;
-        btk_tuning best{};
+        // each candidate's best time over `passes` passes: 3 when a solve's
+        // interactions take under 50 ms (timing noise is then a few percent
+        // of a candidate), 2 under 500 ms, else 1; odd passes run the
+        // candidates in reverse order, so that a drift of the clocks over a
+        // pass favours no candidate
+        std::vector<float> tn(nc.size(), 1e30f), tf(fc.size(), 1e30f);
+        int passes = 1;
         try {
-            for (int i = -1; i < ncall; i++) {  // i = -1 warms up
-                const Shape &a = nc[std::min<int>(std::max(i, 0), (int)nc.size() - 1)];
-                const Shape &b = fc[std::min<int>(std::max(i, 0), (int)fc.size() - 1)];
-                btk_tuning t{};
-                t.near_items = a.items;
-                t.near_tpt = a.tpt;
-                t.mutual = a.mutual;
-                t.far_items = b.items;
-                t.far_tpt = b.tpt;
-                btk::tuning() = t;
-                kev.near_done = kev.far_done = false;
-                btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev);
-                BTK_HIP(hipStreamSynchronize(s));
-                if (i < 0) continue;
-                float ms = 0.f;
-                if (kev.near_done && i < (int)nc.size()) {
-                    BTK_HIP(hipEventElapsedTime(&ms, kev.ev[0], kev.ev[1]));
-                    if (ms < best_near) {
-                        best_near = ms;
-                        best.near_items = a.items;
-                        best.near_tpt = a.tpt;
-                        best.mutual = a.mutual;
+            for (int pass = 0; pass < passes; pass++) {
+                for (int k = pass == 0 ? -1 : 0; k < ncall; k++) {  // k = -1 warms up
+                    const int i = (pass & 1) ? ncall - 1 - k : k;
+                    const Shape &a = nc[std::min<int>(std::max(i, 0), (int)nc.size() - 1)];
+                    const Shape &b = fc[std::min<int>(std::max(i, 0), (int)fc.size() - 1)];
+                    btk_tuning t{};
+                    t.near_items = a.items;
+                    t.near_tpt = a.tpt;
+                    t.mutual = a.mutual;
+                    t.far_items = b.items;
+                    t.far_tpt = b.tpt;
+                    btk::tuning() = t;
+                    kev.near_done = kev.far_done = false;
+                    const auto c0 = std::chrono::steady_clock::now();
+                    btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev);
+                    BTK_HIP(hipStreamSynchronize(s));
+                    if (i < 0) {
+                        const double sec =
+                            std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+                        passes = sec < 0.05 ? 3 : (sec < 0.5 ? 2 : 1);
+                        continue;
                     }
-                }
-                if (kev.far_done && i < (int)fc.size()) {
-                    BTK_HIP(hipEventElapsedTime(&ms, kev.ev[2], kev.ev[3]));
-                    if (ms < best_far) {
-                        best_far = ms;
-                        best.far_items = b.items;
-                        best.far_tpt = b.tpt;
+                    float ms = 0.f;
+                    if (kev.near_done && i < (int)nc.size()) {
+                        BTK_HIP(hipEventElapsedTime(&ms, kev.ev[0], kev.ev[1]));
+                        tn[i] = std::min(tn[i], ms);
+                    }
+                    if (kev.far_done && i < (int)fc.size()) {
+                        BTK_HIP(hipEventElapsedTime(&ms, kev.ev[2], kev.ev[3]));
+                        tf[i] = std::min(tf[i], ms);
                     }
                 }
             }
@@ -636,6 +642,18 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
             throw;
         }
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
+        btk_tuning best{};
+        const size_t in = std::min_element(tn.begin(), tn.end()) - tn.begin();
+        const size_t jf = std::min_element(tf.begin(), tf.end()) - tf.begin();
+        if (tn[in] < 1e30f) {
+            best.near_items = nc[in].items;
+            best.near_tpt = nc[in].tpt;
+            best.mutual = nc[in].mutual;
+        }
+        if (tf[jf] < 1e30f) {
+            best.far_items = fc[jf].items;
+            best.far_tpt = fc[jf].tpt;
+        }
         if (o.precision != BTK_PREC_MIXED) best.near_tpt = best.far_tpt = 0;
         btk::tuning() = best;
         if (chosen) *chosen = best;
