@@ -19,9 +19,10 @@
 // written once.
 //
 // Precision.  FP64: t - s and the kernel in FP64 (the reference's arithmetic).
-// Mixed (F32): t - s to FP32 relative precision (near 3D: FP32 hi/lo pairs,
-// TileSplit; near 2D: formed in FP64 and rounded once; far: relative to the
-// target box centre, TileRel), the kernel in FP32 (hardware log2 / reciprocal
+// Mixed (F32): t - s to FP32 relative precision (near: formed in FP64 and
+// rounded once, or relative to the target box centre for entries of separated
+// boxes, TileMix; far: relative to the target box centre, TileRel), the
+// kernel in FP32 (hardware log2 / reciprocal
 // / reciprocal square root), FP32 partial sums over one 64-source tile, added
 // into FP64 per-target accumulators.
 #pragma once
@@ -74,28 +75,11 @@ struct TileRel {
     float x, y, z, q;
 };
 
-// A source of the 3D near (PP, PC) kernels in mixed precision: each
-// coordinate as an FP32 pair, hi = fl32(x) and lo = fl32(x - hi).  t - s is
-// formed as (t_hi - s_hi) + (t_lo - s_lo): the first difference is exact when
-// the two hi values lie within a factor two of each other (close points) and
-// otherwise rounds relative to t - s itself, the second carries the residuals,
-// so t - s keeps FP32 relative precision down to separations of about
-// 2^-48 |t| -- as forming it in FP64 and rounding does, in three FP32
-// operations instead of an FP64 subtraction and a conversion.  Points closer
-// than that count as coincident under exclude_self.
-struct TileSplit {
-    float xh, yh, zh, q, xl, yl, zl, pad;
-};
-
-__device__ __forceinline__ void split_f32(double x, float &hi, float &lo) {
-    hi = (float)x;
-    lo = (float)(x - (double)hi);
-}
-
 __device__ __forceinline__ double sel3(int d, double a, double b, double c) { return d == 0 ? a : (d == 1 ? b : c); }
 
 // Near pairs of separated boxes take the far field's relative coordinates
-// instead of hi/lo pairs (one FP32 subtraction per axis instead of three).
+// (one FP32 subtraction per axis instead of an FP64 subtraction and a
+// conversion).
 // Relative to the target box centre c, |t - c| + |s - c| <= r_t + d + r_s
 // (d the distance of the centres), while |t - s| >= g, the gap between the
 // boxes; rounding t - c and s - c to FP32 then perturbs t - s by at most
@@ -123,12 +107,13 @@ __device__ __forceinline__ bool separated_for_rel(int T, int S, const TgtView &t
     return g > 0.0 && 0.5 * sqrt(et2) + 0.5 * sqrt(es2) + sqrt(d2) <= kRelFactor * g;
 }
 
-// A source of the 2D near kernels in mixed precision: FP64 coordinates
-// (t - s formed in FP64 and rounded once), or, for entries of separated
-// boxes, FP32 coordinates relative to the target box centre (rx, ry).
-struct TileMix2 {
-    double x, y;
-    float rx, ry, q, pad;
+// A source of the near (PP, PC) kernels in mixed precision: FP64 coordinates
+// (t - s formed in FP64 and rounded once: FP32 relative precision however
+// close the points), or, for entries of separated boxes, FP32 coordinates
+// relative to the target box centre (rx, ry, rz).
+struct TileMix {
+    double x, y, z;
+    float rx, ry, rz, q;
 };
 
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
@@ -203,27 +188,6 @@ __device__ __forceinline__ TileRel load_source_rel(const int4 &m, int j, int n1,
     return TileRel{(float)(a.x - c[0]), (float)(a.y - c[1]), DIM == 3 ? (float)(a.z - c[2]) : 0.0f, a.q};
 }
 
-template <int DIM>
-__device__ __forceinline__ TileSplit load_source_split(const int4 &m, int j, int n1, const NodeTables &tabs,
-                                                       const SrcView &sv) {
-    const TileSrc<true> a = load_source<DIM, true>(m, j, n1, tabs, sv);
-    TileSplit t{};
-    split_f32(a.x, t.xh, t.xl);
-    split_f32(a.y, t.yh, t.yl);
-    split_f32(a.z, t.zh, t.zl);
-    t.q = a.q;
-    return t;
-}
-
-template <int DIM, int KERNEL, bool EXCL>
-__device__ __forceinline__ void eval_pair_split(const TileSplit &s, const float *th, const float *tl, float eps2,
-                                                float *acc32) {
-    const float dx = (th[0] - s.xh) + (tl[0] - s.xl);
-    const float dy = (th[1] - s.yh) + (tl[1] - s.yl);
-    const float dz = DIM == 3 ? (th[2] - s.zh) + (tl[2] - s.zl) : 0.0f;
-    eval_f32<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc32);
-}
-
 template <int DIM, int KERNEL, bool EXCL>
 __device__ __forceinline__ void eval_pair_rel(const TileRel &s, const float *t, float eps2, float *acc32) {
     eval_f32<DIM, KERNEL, EXCL>(t[0] - s.x, t[1] - s.y, DIM == 3 ? t[2] - s.z : 0.0f, s.q, eps2, acc32);
@@ -253,10 +217,8 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
-    constexpr bool SPLIT = F32 && !FAR && DIM == 3;  // FP32 hi/lo coordinates (TileSplit)
-    constexpr bool MIX2 = F32 && !FAR && DIM == 2;   // FP64 or relative FP32 coordinates (TileMix2)
-    using Tile = std::conditional_t<
-        REL, TileRel, std::conditional_t<SPLIT, TileSplit, std::conditional_t<MIX2, TileMix2, TileSrc<F32>>>>;
+    constexpr bool MIX = F32 && !FAR;                // FP64 or relative FP32 coordinates (TileMix)
+    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<MIX, TileMix, TileSrc<F32>>>;
     __shared__ Tile tile[kLanes];
     const int lane = threadIdx.x;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -267,10 +229,10 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
     double t[TPT][3];
-    float tr[TPT][3], th[TPT][3], tl[TPT][3];
+    float tr[TPT][3];
     double acc[TPT][C];
     double ctr[3] = {0.0, 0.0, 0.0};
-    if constexpr (REL || SPLIT || MIX2) {
+    if constexpr (REL || MIX) {
         ctr[0] = 0.5 * (tv.lo0[T] + tv.hi0[T]);
         ctr[1] = 0.5 * (tv.lo1[T] + tv.hi1[T]);
         if constexpr (DIM == 3) ctr[2] = 0.5 * (tv.lo2[T] + tv.hi2[T]);
@@ -288,17 +250,13 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             t[k][2] = DIM == 3 ? tv.sz[ts0 + ti] : 0.0;
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            tr[k][d] = (float)(t[k][d] - ctr[d]);
-            split_f32(t[k][d], th[k][d], tl[k][d]);
-        }
+        for (int d = 0; d < 3; d++) tr[k][d] = (float)(t[k][d] - ctr[d]);
 #pragma unroll
         for (int c = 0; c < C; c++) acc[k][c] = 0.0;
     }
-    // near 3D mixed: entries of separated boxes fill the hi/lo tile with
-    // relative coordinates (hi fields) instead
+    // near, mixed: entries of separated boxes take relative coordinates
     auto rel_entry = [&](const int4 &mm) {
-        if constexpr (SPLIT || MIX2)
+        if constexpr (MIX)
             return separated_for_rel<DIM>(T, mm.x, tv, sv);
         else
             return false;
@@ -306,16 +264,12 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     auto fetch = [&](const int4 &mm, int j, bool rel) -> Tile {
         if constexpr (REL) {
             return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
-        } else if constexpr (SPLIT) {
-            if (rel) {
-                const TileRel r = load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
-                return TileSplit{r.x, r.y, r.z, r.q, 0.0f, 0.0f, 0.0f, 0.0f};
-            }
-            return load_source_split<DIM>(mm, j, n1, tabs, sv);
-        } else if constexpr (MIX2) {
+        } else if constexpr (MIX) {
             const TileSrc<true> a = load_source<DIM, true>(mm, j, n1, tabs, sv);
-            if (rel) return TileMix2{0.0, 0.0, (float)(a.x - ctr[0]), (float)(a.y - ctr[1]), a.q, 0.0f};
-            return TileMix2{a.x, a.y, 0.0f, 0.0f, a.q, 0.0f};
+            if (rel)
+                return TileMix{0.0, 0.0, 0.0, (float)(a.x - ctr[0]), (float)(a.y - ctr[1]),
+                               DIM == 3 ? (float)(a.z - ctr[2]) : 0.0f, a.q};
+            return TileMix{a.x, a.y, DIM == 3 ? a.z : 0.0, 0.0f, 0.0f, 0.0f, a.q};
         } else {
             return load_source<DIM, F32>(mm, j, n1, tabs, sv);
         }
@@ -324,18 +278,13 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
         constexpr bool X = decltype(excl)::value;
         if constexpr (REL)
             eval_pair_rel<DIM, KERNEL, X>(s, tr[k], (float)kp.eps2, a32);
-        else if constexpr (SPLIT) {
+        else if constexpr (MIX) {
             if constexpr (decltype(relc)::value)
-                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.xh, tr[k][1] - s.yh, tr[k][2] - s.zh, s.q, (float)kp.eps2,
-                                         a32);
-            else
-                eval_pair_split<DIM, KERNEL, X>(s, th[k], tl[k], (float)kp.eps2, a32);
-        } else if constexpr (MIX2) {
-            if constexpr (decltype(relc)::value)
-                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.rx, tr[k][1] - s.ry, 0.0f, s.q, (float)kp.eps2, a32);
-            else
-                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f, s.q,
+                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.rx, tr[k][1] - s.ry, DIM == 3 ? tr[k][2] - s.rz : 0.0f, s.q,
                                          (float)kp.eps2, a32);
+            else
+                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y),
+                                         DIM == 3 ? (float)(t[k][2] - s.z) : 0.0f, s.q, (float)kp.eps2, a32);
         } else
             eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
     };
@@ -399,7 +348,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
                 }
             }
         };
-        if constexpr (SPLIT || MIX2) {
+        if constexpr (MIX) {
             if (rel)
                 run_tile(std::true_type{});
             else
