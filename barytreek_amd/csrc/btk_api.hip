@@ -515,23 +515,27 @@ int btk_solve(const btk_run_config *cfg_in, const btk_options *opts, int m, cons
         BTK_HIP(hipStreamSynchronize(s));
         const auto t2 = clk::now();
         const int C = btk::components(dim, o.kernel);
+        // the near field and leaf-to-point accumulate in the target tree's
+        // sorted order (coalesced adds), moved to the caller's order once
         btk::DBuf<double> q(n ? n : 1), W(btk::proxy_offset(*l, ts->pc, 1, l->ndeg)),
-            Psi(btk::proxy_offset(*l, tt.pc, C, l->ndeg));
+            Psi(btk::proxy_offset(*l, tt.pc, C, l->ndeg)), outs((size_t)C * (m ? m : 1));
         btk::gather_sorted(*ts, d_q, q.p, s);
         btk::upward_all(cfg, *l, *ts, q.p, W.p, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
+        BTK_HIP(hipMemsetAsync(outs.p, 0, sizeof(double) * outs.n, s));
         btk::KernelEvents kev;
         for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
-        btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, d_out, Psi.p, s, &kev);
+        btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, outs.p, Psi.p, s, &kev, true);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t4 = clk::now();
         float ms_near = 0.f, ms_far = 0.f;
         if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
         if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
-        btk::downward_all(cfg, o, *l, tt, Psi.p, d_out, s);
+        btk::downward_all(cfg, o, *l, tt, Psi.p, outs.p, s, true);
+        btk::add_unsorted(tt, C, outs.p, d_out, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
         if (stats) {
@@ -804,22 +808,24 @@ int btk_solve_batch(const btk_run_config *cfg_in, const btk_options *opts, int n
         const int C = btk::components(dim, o.kernel);
         const int n = t->n;
         btk::DBuf<double> q(n ? n : 1), W(btk::proxy_offset(*l, t->pc, 1, l->ndeg)),
-            Psi(btk::proxy_offset(*l, t->pc, C, l->ndeg));
+            Psi(btk::proxy_offset(*l, t->pc, C, l->ndeg)), outs((size_t)C * (n ? n : 1));
         btk::gather_sorted(*t, d_q, q.p, s);
         btk::upward_all(cfg, *l, *t, q.p, W.p, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t3 = clk::now();
         BTK_HIP(hipMemsetAsync(Psi.p, 0, sizeof(double) * Psi.n, s));
+        BTK_HIP(hipMemsetAsync(outs.p, 0, sizeof(double) * outs.n, s));
         btk::KernelEvents kev;
         for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
-        btk::fmm_interactions(cfg, o, *t, *t, *l, q.p, W.p, d_out, Psi.p, s, &kev);
+        btk::fmm_interactions(cfg, o, *t, *t, *l, q.p, W.p, outs.p, Psi.p, s, &kev, true);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t4 = clk::now();
         float ms_near = 0.f, ms_far = 0.f;
         if (kev.near_done) BTK_HIP(hipEventElapsedTime(&ms_near, kev.ev[0], kev.ev[1]));
         if (kev.far_done) BTK_HIP(hipEventElapsedTime(&ms_far, kev.ev[2], kev.ev[3]));
         for (auto &e : kev.ev) (void)hipEventDestroy(e);
-        btk::downward_all(cfg, o, *l, *t, Psi.p, d_out, s);
+        btk::downward_all(cfg, o, *l, *t, Psi.p, outs.p, s, true);
+        btk::add_unsorted(*t, C, outs.p, d_out, s);
         BTK_HIP(hipStreamSynchronize(s));
         const auto t5 = clk::now();
         if (stats) {

@@ -154,7 +154,7 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             InteractLaunch L;
             L.tk = TaskView{g.tk.panel.p, g.tk.first.p, g.tk.e0.p, g.tk.e1.p};
             L.ev = EntryView{p.meta.p};
-            L.tv = TgtView{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
+            L.tv = TgtView{tt.start.p, tt.count.p, p.sorted_out ? nullptr : tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
                            tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
             L.sv = SrcView{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
                            ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W + p.woff[di]};
@@ -171,7 +171,7 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.kp = p.kp;
             L.out = dst;
             L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
-            if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, s);
+            if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, p.sorted_out, s);
             if (g.cc.n_items)
                 launch_ccgemm(g.cc, tt, ts, W + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride, s);
             if (g.tk.n) launch_interact<DIM, KERNEL>(L, s);
@@ -372,11 +372,12 @@ void launch_interactions(const InteractPlan &p, const double *q_sorted, const do
 
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts, const List &l,
                       const double *q_sorted, const double *W, double *out, double *Psi, hipStream_t s,
-                      KernelEvents *kev) {
+                      KernelEvents *kev, bool sorted_out) {
     // the plan's buffers go back to the cache on return, while the launches
     // may still run; the cache hands them out only to this thread's later
     // work on this stream until the API call has synchronized
     const auto p = plan_interactions(cfg, o, tt, ts, l, s);
+    p->sorted_out = sorted_out;
     launch_interactions(*p, q_sorted, W, out, Psi, s, kev);
 }
 

@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     for (int k = 0; k < TPT; k++) {
         const int ti = first + lane + kLanes * k;
         if (ti < ntgt) {
-            const size_t o = FAR ? (size_t)T * P + ti : (size_t)tv.perm[ts0 + ti];
+            const size_t o = FAR ? (size_t)T * P + ti : (size_t)(tv.perm ? tv.perm[ts0 + ti] : ts0 + ti);
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 double scale = comp_scale<DIM, KERNEL>(c, kp.gfc);
@@ -436,20 +436,21 @@ struct Mutual {
 // Finds the mutual pairs of a one-tree list, sets the source count of both
 // their entries in meta to zero and builds the work items.
 void build_mutual(const List &l, const Tree &t, int tpt, int4 *meta, Mutual &mu, hipStream_t s);
-// Mixed-precision evaluation of the mutual pairs, both directions, into out[C][n].
+// Mixed-precision evaluation of the mutual pairs, both directions, into out[C][n]
+// (in the tree's sorted order when sorted_out, else in the original order).
 template <int DIM, int KERNEL>
 void launch_mutual(const Mutual &mu, const Tree &t, const double *q_sorted, const KParams &kp, double *out,
-                   hipStream_t s);
+                   bool sorted_out, hipStream_t s);
 extern template void launch_mutual<2, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &,
-                                                      double *, hipStream_t);
+                                                      double *, bool, hipStream_t);
 extern template void launch_mutual<2, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
-                                                           const KParams &, double *, hipStream_t);
+                                                           const KParams &, double *, bool, hipStream_t);
 extern template void launch_mutual<3, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &,
-                                                      double *, hipStream_t);
+                                                      double *, bool, hipStream_t);
 extern template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, const Tree &, const double *,
-                                                            const KParams &, double *, hipStream_t);
+                                                            const KParams &, double *, bool, hipStream_t);
 extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *,
-                                                           const KParams &, double *, hipStream_t);
+                                                           const KParams &, double *, bool, hipStream_t);
 
 // CC pairs sharing a kernel matrix (btk_mrhs.hip), mixed precision, 3D.
 // Panel boxes are exact halves of their parents, so the P x P kernel matrix
@@ -500,6 +501,7 @@ struct InteractPlan {
         CCGemm cc;
     } g[2][4];  // [far][degree index]
     long long cc_shared = 0;  // CC entries evaluated through shared kernel matrices
+    bool sorted_out = false;  // near output in the target tree's sorted order
 };
 
 // Launch of one interaction group with everything fixed but the kernel.

@@ -214,9 +214,11 @@ struct KernelEvents {
 };
 // W and Psi hold every degree of the list (proxy_offset with 1 and C
 // components); with one degree they are the plain arrays.
+// sorted_out: the near-field results go to out in the target tree's sorted
+// order (coalesced adds; add_unsorted moves them to the original order).
 void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
                       const List &l, const double *q_sorted, const double *W, double *out, double *Psi,
-                      hipStream_t s, KernelEvents *kev = nullptr);
+                      hipStream_t s, KernelEvents *kev = nullptr, bool sorted_out = false);
 // The same in two steps: the plan (descriptors and work items; allocates and
 // reads counts back) and the launches alone.  The plan refers to tt, ts.
 struct InteractPlan;
@@ -225,7 +227,9 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
 void launch_interactions(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
                          hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
-                   hipStream_t s);
+                   hipStream_t s, bool sorted_out = false);
+// out[c][perm[i]] += sorted[c][i] over the tree's n points and C components.
+void add_unsorted(const Tree &t, int C, const double *sorted, double *out, hipStream_t s);
 // The passes over every degree of a list.  upward_all runs the upward pass at
 // the configured degree and restricts its charges to each lower degree, which
 // equals that degree's own upward pass: degree-d basis functions are
@@ -236,7 +240,7 @@ void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &
 void upward_all(const btk_run_config &cfg, const List &l, const Tree &src, const double *q_sorted, double *W,
                 hipStream_t s);
 void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l, const Tree &tt, double *Psi,
-                  double *out, hipStream_t s);
+                  double *out, hipStream_t s, bool sorted_out = false);
 
 // Many charge vectors (btk_mrhs.hip).  q is [K][n] in original order;
 // qs is [n][kp] FP32 in sorted order, kp = K rounded up to 64, zero padded.

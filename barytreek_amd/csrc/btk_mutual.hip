@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(kLanes * kMaxWaves)
                 if (j < nS) {
                     float sum = 0.0f;
                     for (int v = 0; v < W; v++) sum += red[c][v][jj];
-                    unsafeAtomicAdd(out + c * comp_stride + (size_t)perm[s0 + j],
+                    unsafeAtomicAdd(out + c * comp_stride + (size_t)(perm ? perm[s0 + j] : s0 + j),
                                     comp_scale_f32<DIM, KERNEL>(c, kp.gfc) * (double)sum);
                 }
             }
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kLanes * kMaxWaves)
     for (int k = 0; k < TPT; k++) {
         const int ti = first + lane + kLanes * k;
         if (ti < nT) {
-            const size_t o = (size_t)perm[t0 + ti];
+            const size_t o = (size_t)(perm ? perm[t0 + ti] : t0 + ti);
 #pragma unroll
             for (int c = 0; c < C; c++)
                 unsafeAtomicAdd(out + c * comp_stride + o, comp_scale_f32<DIM, KERNEL>(c, kp.gfc) * acc[k][c]);
@@ -290,13 +290,14 @@ void build_mutual(const List &l, const Tree &t, int tpt, int4 *meta, Mutual &mu,
 
 template <int DIM, int KERNEL>
 void launch_mutual(const Mutual &mu, const Tree &t, const double *q_sorted, const KParams &kp, double *out,
-                   hipStream_t s) {
+                   bool sorted_out, hipStream_t s) {
     if (mu.n <= 0) return;
     const size_t stride = (size_t)t.n;
     const TaskView tk{mu.tasks.panel.p, mu.tasks.first.p, mu.tasks.e0.p, mu.tasks.e1.p};
     const EntryView mv{mu.mmeta.p};
+    const int *perm = sorted_out ? nullptr : t.perm.p;
 #define BTK_MU(TPT_)                                                                                          \
-    k_mutual<DIM, KERNEL, TPT_><<<mu.n, kLanes * mu.waves, 0, s>>>(tk, mv, t.start.p, t.count.p, t.perm.p,    \
+    k_mutual<DIM, KERNEL, TPT_><<<mu.n, kLanes * mu.waves, 0, s>>>(tk, mv, t.start.p, t.count.p, perm,       \
                                                                     t.sx.p, t.sy.p, t.sz.p, q_sorted, kp, out, \
                                                                     stride)
     if (mu.tpt == 2)
@@ -308,14 +309,14 @@ void launch_mutual(const Mutual &mu, const Tree &t, const double *q_sorted, cons
 }
 
 template void launch_mutual<2, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &, double *,
-                                               hipStream_t);
+                                               bool, hipStream_t);
 template void launch_mutual<2, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *, const KParams &,
-                                                    double *, hipStream_t);
+                                                    double *, bool, hipStream_t);
 template void launch_mutual<3, BTK_KERNEL_POT>(const Mutual &, const Tree &, const double *, const KParams &, double *,
-                                               hipStream_t);
+                                               bool, hipStream_t);
 template void launch_mutual<3, BTK_KERNEL_REF_VEL3D>(const Mutual &, const Tree &, const double *, const KParams &,
-                                                     double *, hipStream_t);
+                                                     double *, bool, hipStream_t);
 template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const Tree &, const double *, const KParams &,
-                                                    double *, hipStream_t);
+                                                    double *, bool, hipStream_t);
 
 }  // namespace btk

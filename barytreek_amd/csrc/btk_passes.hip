@@ -266,6 +266,17 @@ __global__ void __launch_bounds__(TB)
     }
 }
 
+// out[c][perm[i]] += sorted[c][i]: results accumulated in the tree's sorted
+// order (coalesced) moved to the caller's order once.  Each output element
+// has one source element, so plain adds suffice.
+__global__ void k_add_unsorted(int n, long long total, const int *__restrict__ perm,
+                               const double *__restrict__ sorted, double *__restrict__ out) {
+    const long long k = (long long)blockIdx.x * TB + threadIdx.x;
+    if (k >= total) return;
+    const long long c = k / n, i = k % n;
+    out[c * n + perm[i]] += sorted[k];
+}
+
 // ---------------------------------------------------------------------------
 // leaf -> point (downward_pass_impl.hpp:207-236): one workgroup per leaf.
 // ---------------------------------------------------------------------------
@@ -289,7 +300,7 @@ __global__ void __launch_bounds__(TB)
         bli_basis<N1>(sx[s0 + i], lo0, hi0, tabs.a, N1, Lx);
         bli_basis<N1>(sy[s0 + i], lo1, hi1, tabs.a, N1, Ly);
         if constexpr (DIM == 3) bli_basis<N1>(sz[s0 + i], bx.lo2[leaf], bx.hi2[leaf], tabs.a, N1, Lz);
-        const int o = perm[s0 + i];
+        const int o = perm ? perm[s0 + i] : s0 + i;
         for (int c = 0; c < ncomp; c++) {
             double phi = 0.0;
             if constexpr (DIM == 3) {
@@ -338,11 +349,11 @@ void launch_anterp(int n1, int nleaves, const Tree &t, const double *q, const No
 
 template <int DIM>
 void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const NodeTables &tabs, int ncomp,
-                size_t stride, const double *Psi, double *out, hipStream_t s) {
+                size_t stride, const int *perm, const double *Psi, double *out, hipStream_t s) {
 #define BTK_L2P(N)                                                                                              \
     case N:                                                                                                     \
         k_l2p<DIM, N><<<nleaves, TB, 0, s>>>(leaves, t.start.p, t.count.p, boxes(t), t.sx.p, t.sy.p, t.sz.p,    \
-                                             t.perm.p, tabs, ncomp, stride, Psi, out, t.n);                     \
+                                             perm, tabs, ncomp, stride, Psi, out, t.n);                         \
         break;
     switch (n1) {
         BTK_L2P(2) BTK_L2P(3) BTK_L2P(4) BTK_L2P(5) BTK_L2P(6) BTK_L2P(7) BTK_L2P(8) BTK_L2P(9) BTK_L2P(10)
@@ -447,7 +458,7 @@ void upward_pass(const btk_run_config &cfg, const Tree &t, const double *q_sorte
 }
 
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &t, double *Psi, double *out,
-                   hipStream_t s) {
+                   hipStream_t s, bool sorted_out) {
     const int n1 = cfg.interp_degree + 1;
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
     const int ncomp = components(t.dim, o.kernel);
@@ -468,10 +479,18 @@ void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &
     const int nl = o.deepest_only ? t.n_leaves_deepest : t.n_leaves;
     const int *lv = t.leaves.p + (t.n_leaves - nl);
     if (nl == 0) return;
+    const int *perm = sorted_out ? nullptr : t.perm.p;
     if (t.dim == 3)
-        launch_l2p<3>(n1, lv, nl, t, tabs, ncomp, stride, Psi, out, s);
+        launch_l2p<3>(n1, lv, nl, t, tabs, ncomp, stride, perm, Psi, out, s);
     else
-        launch_l2p<2>(n1, lv, nl, t, tabs, ncomp, stride, Psi, out, s);
+        launch_l2p<2>(n1, lv, nl, t, tabs, ncomp, stride, perm, Psi, out, s);
+}
+
+void add_unsorted(const Tree &t, int C, const double *sorted, double *out, hipStream_t s) {
+    const long long total = (long long)C * t.n;
+    if (total == 0) return;
+    k_add_unsorted<<<nblk(total), TB, 0, s>>>(t.n, total, t.perm.p, sorted, out);
+    BTK_LAUNCH_CHECK();
 }
 
 }  // namespace btk
@@ -502,7 +521,7 @@ void upward_all(const btk_run_config &cfg, const List &l, const Tree &t, const d
 }
 
 void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l, const Tree &t, double *Psi,
-                  double *out, hipStream_t s) {
+                  double *out, hipStream_t s, bool sorted_out) {
     const int top = l.ndeg - 1, n = l.deg[top];
     BTK_REQUIRE(n == cfg.interp_degree, "the list was built for another interp_degree");
     const int C = components(t.dim, o.kernel);
@@ -517,7 +536,7 @@ void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l
             BTK_LAUNCH_CHECK();
         }
     }
-    downward_pass(cfg, o, t, Pn, out, s);
+    downward_pass(cfg, o, t, Pn, out, s, sorted_out);
 }
 
 }  // namespace btk
