@@ -24,7 +24,7 @@ struct btk_plan {
     const btk::Tree *tt = nullptr;
     std::unique_ptr<btk::List> l;
     std::shared_ptr<btk::InteractPlan> ip;
-    btk::DBuf<double> q, W, Psi;
+    btk::DBuf<double> q, W, Psi, outs;  // outs: results in the target tree's sorted order
     hipStream_t cs = nullptr;  // capture stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -621,7 +621,7 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
                     btk::tuning() = t;
                     kev.near_done = kev.far_done = false;
                     const auto c0 = std::chrono::steady_clock::now();
-                    btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev);
+                    btk::fmm_interactions(cfg, o, tt, *ts, *l, q.p, W.p, out.p, Psi.p, s, &kev, true);
                     BTK_HIP(hipStreamSynchronize(s));
                     if (i < 0) {
                         const double sec =
@@ -731,13 +731,14 @@ int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m
         if (!coincident) p->tt_own = btk::build_tree(dim, p->cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
         p->tt = coincident ? p->ts.get() : p->tt_own.get();
         p->l = btk::build_list(&p->cfg, *p->tt, *p->ts, s, p->o.mac != 0);
-        p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s);
+        p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s, true);
         p->m = m;
         p->n = n;
         p->C = btk::components(dim, p->o.kernel);
         p->q.resize(n ? n : 1);
         p->W.resize(btk::proxy_offset(*p->l, p->ts->pc, 1, p->l->ndeg));
         p->Psi.resize(btk::proxy_offset(*p->l, p->tt->pc, p->C, p->l->ndeg));
+        p->outs.resize((size_t)p->C * (m ? m : 1));
         BTK_HIP(hipStreamCreateWithFlags(&p->cs, hipStreamNonBlocking));
         BTK_HIP(hipStreamSynchronize(s));
         *plan = p.release();
@@ -756,8 +757,10 @@ int btk_plan_execute(btk_plan *p, const double *d_q, double *d_out, void *stream
                 btk::gather_sorted(*p->ts, d_q, p->q.p, p->cs);
                 btk::upward_all(p->cfg, *p->l, *p->ts, p->q.p, p->W.p, p->cs);
                 btk::zero_fill(p->Psi.p, p->Psi.n, p->cs);
-                btk::launch_interactions(*p->ip, p->q.p, p->W.p, d_out, p->Psi.p, p->cs);
-                btk::downward_all(p->cfg, p->o, *p->l, *p->tt, p->Psi.p, d_out, p->cs);
+                btk::zero_fill(p->outs.p, p->outs.n, p->cs);
+                btk::launch_interactions(*p->ip, p->q.p, p->W.p, p->outs.p, p->Psi.p, p->cs);
+                btk::downward_all(p->cfg, p->o, *p->l, *p->tt, p->Psi.p, p->outs.p, p->cs, true);
+                btk::add_unsorted(*p->tt, p->C, p->outs.p, d_out, p->cs);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(p->cs, &g);

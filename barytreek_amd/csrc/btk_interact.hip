@@ -280,7 +280,7 @@ int choose_split(long long base_items, long long target_items) {
 }
 
 std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt,
-                                                const Tree &ts, const List &l, hipStream_t s) {
+                                                const Tree &ts, const List &l, hipStream_t s, bool sorted_out) {
     BTK_REQUIRE(tt.dim == ts.dim, "target and source trees differ in dimension");
     const int dim = tt.dim;
     const int n1 = cfg.interp_degree + 1;
@@ -292,6 +292,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
     auto p = std::make_shared<InteractPlan>();
     p->cfg = cfg;
     p->o = o;
+    p->sorted_out = sorted_out;
     p->tt = &tt;
     p->ts = &ts;
     p->kp = make_kparams(dim, cfg.ker_eps, o);
@@ -376,8 +377,7 @@ void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tre
     // the plan's buffers go back to the cache on return, while the launches
     // may still run; the cache hands them out only to this thread's later
     // work on this stream until the API call has synchronized
-    const auto p = plan_interactions(cfg, o, tt, ts, l, s);
-    p->sorted_out = sorted_out;
+    const auto p = plan_interactions(cfg, o, tt, ts, l, s, sorted_out);
     launch_interactions(*p, q_sorted, W, out, Psi, s, kev);
 }
 

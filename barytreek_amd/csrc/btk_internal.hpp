@@ -223,7 +223,8 @@ void fmm_interactions(const btk_run_config &cfg, const btk_options &o, const Tre
 // reads counts back) and the launches alone.  The plan refers to tt, ts.
 struct InteractPlan;
 std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt,
-                                                const Tree &ts, const List &l, hipStream_t s);
+                                                const Tree &ts, const List &l, hipStream_t s,
+                                                bool sorted_out = false);
 void launch_interactions(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
                          hipStream_t s, KernelEvents *kev = nullptr);
 void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &tt, double *Psi, double *out,
