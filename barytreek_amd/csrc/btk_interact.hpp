@@ -142,14 +142,11 @@ __device__ __forceinline__ void proxy_point(int idx, int n1, const NodeTables &t
     }
 }
 
-template <int DIM, int KERNEL, bool F32, bool EXCL>
-__device__ __forceinline__ void eval_pair(const TileSrc<F32> &s, const double *t, double eps2, double *acc64,
-                                          float *acc32) {
+// The FP64 path: t - s and the kernel in FP64.
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_pair(const TileSrc<false> &s, const double *t, double eps2, double *acc64) {
     const double dx = t[0] - s.x, dy = t[1] - s.y, dz = DIM == 3 ? t[2] - s.z : 0.0;
-    if constexpr (F32)
-        eval_f32<DIM, KERNEL, EXCL>((float)dx, (float)dy, (float)dz, s.q, (float)eps2, acc32);
-    else
-        eval_f64<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc64);
+    eval_f64<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc64);
 }
 
 // Source j of entry m: a point of a source leaf, or a proxy point of a
@@ -285,8 +282,10 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             else
                 eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y),
                                          DIM == 3 ? (float)(t[k][2] - s.z) : 0.0f, s.q, (float)kp.eps2, a32);
-        } else
-            eval_pair<DIM, KERNEL, F32, X>(s, t[k], kp.eps2, acc[k], a32);
+        } else {
+            static_assert(!F32, "mixed precision takes TileRel (far) or TileMix (near)");
+            eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k]);
+        }
     };
     // The entries' sources stream through the tile 64 at a time, as one
     // sequence over all entries of the item; the next tile's loads are
