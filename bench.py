@@ -416,8 +416,8 @@ def bench_forest(args, btk, torch, wl, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by target "
-              f"subtrees, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (a multi-GPU solve would shard "
+              f"the targets by subtrees, after an all-gather of the upper levels' proxy weights)", file=sys.stderr)
         sys.exit(2)
     import torch
     import barytreek_amd as btk
