@@ -146,6 +146,46 @@ __global__ void k_level_flags(int ls, int np, int thresh, const int *__restrict_
     val[i] = f | (nch << 32);
 }
 
+// The same flags, their inclusive scan and its total (written to host memory)
+// in one workgroup, for levels of at most kLevelFuse panels: one launch and
+// one readback instead of the flags, a device-wide scan and a copy.
+constexpr int kLevelFuse = 8192;
+constexpr int kLevelThreads = 1024;
+__global__ void __launch_bounds__(kLevelThreads)
+    k_level_scan(int ls, int np, int thresh, const int *__restrict__ count, long long *__restrict__ val,
+                 long long *__restrict__ inc, long long *host_total) {
+    __shared__ long long wsum[kLevelThreads / 64];
+    __shared__ long long carry;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < np; base += kLevelThreads) {
+        const int i = base + tid;
+        long long v = 0;
+        if (i < np) {
+            const int c = count[ls + i];
+            const long long f = c > thresh ? 1 : 0;
+            v = f | ((f ? (long long)((c + CHUNK - 1) / CHUNK) : 0) << 32);
+            val[i] = v;
+        }
+        long long x = v;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        long long pre = carry;
+        for (int k = 0; k < w; k++) pre += wsum[k];
+        if (i < np) inc[i] = pre + x;
+        __syncthreads();  // every thread has read carry and wsum
+        if (tid == kLevelThreads - 1) carry = pre + x;
+        __syncthreads();
+    }
+    if (tid == 0) *host_total = carry;
+}
+
 // Child boxes (tree_construction.cpp:184-194): "upper" halves are [mid, max].
 __constant__ int c_hx[8] = {1, 0, 0, 1, 1, 0, 0, 1};
 __constant__ int c_hy[8] = {1, 1, 0, 0, 1, 1, 0, 0};
@@ -465,10 +505,15 @@ void build_levels(Tree &t, hipStream_t s) {
         const int ls = t.level_start[lvl];
         const int np = t.pc - ls;
         DBuf<long long> val(np), inc(np);
-        k_level_flags<<<nblk(np), TB, 0, s>>>(ls, np, t.thresh, t.count.p, val.p);
-        BTK_LAUNCH_CHECK();
-        inclusive_sum_i64(val.p, inc.p, np, s);
-        BTK_HIP(hipMemcpyAsync(hv, inc.p + (np - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
+        if (np <= kLevelFuse) {
+            k_level_scan<<<1, kLevelThreads, 0, s>>>(ls, np, t.thresh, t.count.p, val.p, inc.p, hv);
+            BTK_LAUNCH_CHECK();
+        } else {
+            k_level_flags<<<nblk(np), TB, 0, s>>>(ls, np, t.thresh, t.count.p, val.p);
+            BTK_LAUNCH_CHECK();
+            inclusive_sum_i64(val.p, inc.p, np, s);
+            BTK_HIP(hipMemcpyAsync(hv, inc.p + (np - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
+        }
         BTK_HIP(hipStreamSynchronize(s));
         const long long tot = hv[0];
         const int R = (int)(tot & 0xffffffffLL);
