@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
                     KParams kp, int M, int pct, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     __shared__ double sc[3][64];
+    __shared__ float scr[3][64];  // relative to the target box centre (separated boxes)
     __shared__ float qt[64][KC + 1];
     __shared__ double cn[3][kMaxN1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -92,6 +93,11 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
         t[1] = tv.sy[ts0 + tc];
         t[2] = DIM == 3 ? tv.sz[ts0 + tc] : 0.0;
     }
+    // far pairs, and near pairs of separated boxes, take FP32 coordinates
+    // relative to the target box centre (the error bound of the one-vector kernels)
+    const double ctr[3] = {0.5 * (tv.lo0[T] + tv.hi0[T]), 0.5 * (tv.lo1[T] + tv.hi1[T]),
+                           DIM == 3 ? 0.5 * (tv.lo2[T] + tv.hi2[T]) : 0.0};
+    const float tr[3] = {(float)(t[0] - ctr[0]), (float)(t[1] - ctr[1]), (float)(t[2] - ctr[2])};
     const int col = 32 * (w >> 1) + (lane & 31);
     const int par = lane >> 5;
     double acc[C][16];
@@ -112,7 +118,8 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
             }
             __syncthreads();
         }
-        const bool check = EXCL && (!same_tree || FAR || cluster || S == T);
+        const bool rel = FAR || separated_for_rel<DIM>(T, S, tv, sv);
+        const bool check = EXCL && !rel && (!same_tree || FAR || cluster || S == T);
         const int ss = cluster ? 0 : sv.start[S];
         for (int t0 = 0; t0 < nsrc; t0 += 64) {
             const int nt = min(64, nsrc - t0);
@@ -136,9 +143,15 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
                         z = DIM == 3 ? sv.sz[ss + j] : 0.0;
                     }
                 }
-                sc[0][tid] = x;
-                sc[1][tid] = y;
-                sc[2][tid] = z;
+                if (rel) {
+                    scr[0][tid] = (float)(x - ctr[0]);
+                    scr[1][tid] = (float)(y - ctr[1]);
+                    scr[2][tid] = (float)(z - ctr[2]);
+                } else {
+                    sc[0][tid] = x;
+                    sc[1][tid] = y;
+                    sc[2][tid] = z;
+                }
             }
             const float *qsrc = cluster ? mv.w + ((size_t)S * P + t0) * mv.kp : mv.q + (size_t)(ss + t0) * mv.kp;
             for (int k = tid; k < 64 * KC; k += TB) {
@@ -149,12 +162,20 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
             f32x16 d[C];
 #pragma unroll
             for (int c = 0; c < C; c++) d[c] = (f32x16){0};
+            auto steps = [&](auto relc) {
 #pragma unroll 4
             for (int step = 0; step < 32; step++) {
                 const int s = 2 * step + par;
-                const float dx = (float)(t[0] - sc[0][s]);
-                const float dy = (float)(t[1] - sc[1][s]);
-                const float dz = DIM == 3 ? (float)(t[2] - sc[2][s]) : 0.0f;
+                float dx, dy, dz;
+                if constexpr (decltype(relc)::value) {
+                    dx = tr[0] - scr[0][s];
+                    dy = tr[1] - scr[1][s];
+                    dz = DIM == 3 ? tr[2] - scr[2][s] : 0.0f;
+                } else {
+                    dx = (float)(t[0] - sc[0][s]);
+                    dy = (float)(t[1] - sc[1][s]);
+                    dz = DIM == 3 ? (float)(t[2] - sc[2][s]) : 0.0f;
+                }
                 float g[C];
                 if (check)
                     kernel_row<DIM, KERNEL, true>(dx, dy, dz, eps2, g);
@@ -168,6 +189,11 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
 #pragma unroll
                 for (int c = 0; c < C; c++) d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(g[c], b, d[c], 0, 0, 0);
             }
+            };
+            if (rel)
+                steps(std::true_type{});
+            else
+                steps(std::false_type{});
 #pragma unroll
             for (int c = 0; c < C; c++)
 #pragma unroll
