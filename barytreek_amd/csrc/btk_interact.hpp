@@ -111,9 +111,17 @@ __device__ __forceinline__ bool separated_for_rel(int T, int S, const TgtView &t
 // (t - s formed in FP64 and rounded once: FP32 relative precision however
 // close the points), or, for entries of separated boxes, FP32 coordinates
 // relative to the target box centre (rx, ry, rz).
+// Laid out for aligned LDS reads: the relative form (rx, ry, rz, q) in one
+// 16-byte load, x and y in another; 2D drops z.
+template <int DIM>
 struct TileMix {
-    double x, y, z;
     float rx, ry, rz, q;
+    double x, y;
+};
+template <>
+struct TileMix<3> {
+    float rx, ry, rz, q;
+    double x, y, z, pad;
 };
 
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
@@ -215,7 +223,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
     constexpr bool MIX = F32 && !FAR;                // FP64 or relative FP32 coordinates (TileMix)
-    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<MIX, TileMix, TileSrc<F32>>>;
+    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<MIX, TileMix<DIM>, TileSrc<F32>>>;
     __shared__ Tile tile[kLanes];
     const int lane = threadIdx.x;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -263,10 +271,20 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             return load_source_rel<DIM>(mm, j, n1, tabs, sv, ctr);
         } else if constexpr (MIX) {
             const TileSrc<true> a = load_source<DIM, true>(mm, j, n1, tabs, sv);
-            if (rel)
-                return TileMix{0.0, 0.0, 0.0, (float)(a.x - ctr[0]), (float)(a.y - ctr[1]),
-                               DIM == 3 ? (float)(a.z - ctr[2]) : 0.0f, a.q};
-            return TileMix{a.x, a.y, DIM == 3 ? a.z : 0.0, 0.0f, 0.0f, 0.0f, a.q};
+            {
+                Tile r{};
+                r.q = a.q;
+                if (rel) {
+                    r.rx = (float)(a.x - ctr[0]);
+                    r.ry = (float)(a.y - ctr[1]);
+                    r.rz = DIM == 3 ? (float)(a.z - ctr[2]) : 0.0f;
+                } else {
+                    r.x = a.x;
+                    r.y = a.y;
+                    if constexpr (DIM == 3) r.z = a.z;
+                }
+                return r;
+            }
         } else {
             return load_source<DIM, F32>(mm, j, n1, tabs, sv);
         }
@@ -279,9 +297,12 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             if constexpr (decltype(relc)::value)
                 eval_f32<DIM, KERNEL, X>(tr[k][0] - s.rx, tr[k][1] - s.ry, DIM == 3 ? tr[k][2] - s.rz : 0.0f, s.q,
                                          (float)kp.eps2, a32);
+            else if constexpr (DIM == 3)
+                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), (float)(t[k][2] - s.z),
+                                         s.q, (float)kp.eps2, a32);
             else
-                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y),
-                                         DIM == 3 ? (float)(t[k][2] - s.z) : 0.0f, s.q, (float)kp.eps2, a32);
+                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f, s.q,
+                                         (float)kp.eps2, a32);
         } else {
             static_assert(!F32, "mixed precision takes TileRel (far) or TileMix (near)");
             eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k]);
