@@ -1,7 +1,8 @@
 """Edge cases of the path against the oracle: empty and tiny inputs, points
 on the splitting planes (the strict '>' rule), more coincident points than a
 leaf holds, the lowest and highest interpolation degrees, and a MAC that
-accepts nothing (the treecode is then the direct sum)."""
+accepts nothing (the treecode is then the direct sum); the solve with separate
+target and source points and with the reference's leaf set for leaf-to-point."""
 import numpy as np
 import pytest
 
@@ -149,3 +150,42 @@ def test_mac_accepting_nothing_is_the_direct_sum(dim):
     e = float(torch.linalg.norm(out - ds) / torch.linalg.norm(ds))
     print(f"\n[edges] theta 0.01 dim={dim}: {int(st.count[0])} PP pairs, rel vs direct sum {e:.2e}", flush=True)
     assert e < 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_solve_with_separate_targets_matches_oracle(dim):
+    """btk_solve with distinct target and source points (two trees,
+    example_poisson_2d_split.cpp:80-170): the results, accumulated in the
+    target tree's sorted order and moved to the caller's, equal the oracle's."""
+    xt, yt, zt, _ = wl.uniform_cube(9000, dim, 70 + dim)
+    xs, ys, zs, q = wl.uniform_cube(7000, dim, 80 + dim)
+    xs, ys = 0.8 * xs + 0.1, 0.8 * ys - 0.1
+    cfg = btk.make_config(dim, 6, 120, 0.7, 1e-3)
+    o = btk.options(kernel=btk.KERNEL_POT_GRAD)
+    Cn = btk.components(dim, btk.KERNEL_POT_GRAD)
+    out = torch.zeros(Cn * len(xt), dtype=torch.float64, device="cuda")
+    btk.solve(cfg, o, dev(xt), dev(yt), dev(zt), dev(q), out, xs=dev(xs), ys=dev(ys), zs=dev(zs))
+    O.set_threads(16)
+    oout, _, _ = O.solve(O.make_config(dim, 6, 120, 0.7, 1e-3), O.opts(kernel=O.KERNEL_POT_GRAD), xt, yt, zt,
+                         xs, ys, zs, q, coincident=False)
+    e = rel(out.cpu().numpy(), oout)
+    print(f"\n[edges] separate targets dim={dim}: rel {e:.2e}", flush=True)
+    assert e < TOL
+
+
+def test_solve_deepest_only_matches_oracle():
+    """btk_solve with the reference's leaf set for leaf-to-point (F5,
+    downward_pass.cpp:14-16) on a tree of uneven depth."""
+    x, y, z, q = wl.sphere_plummer_3d(12000, 91)
+    cfg = btk.make_config(3, 5, 150, 0.7, 1e-4)
+    o = btk.options(kernel=btk.KERNEL_POT)
+    o.deepest_only = 1
+    out = torch.zeros(len(x), dtype=torch.float64, device="cuda")
+    btk.solve(cfg, o, dev(x), dev(y), dev(z), dev(q), out)
+    oo = O.opts(kernel=O.KERNEL_POT)
+    oo.deepest_only = 1
+    O.set_threads(16)
+    oout, _, _ = O.solve(O.make_config(3, 5, 150, 0.7, 1e-4), oo, x, y, z, x, y, z, q)
+    e = rel(out.cpu().numpy(), oout)
+    print(f"\n[edges] deepest_only solve: rel {e:.2e}", flush=True)
+    assert e < TOL
