@@ -18,7 +18,12 @@ sampled targets above).  Launch shapes are autotuned before the warmup.
                                 3d-batched-2M|3d-forest-256]
                     [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
                     [--no-cpu-baseline] [--cpu-stride K] [--no-accuracy] [--no-degree-scan]
-                    [--mac-budget B] [--n N] [--leaf L] [--degree D]
+                    [--mac-budget B] [--mac-sets S] [--mac-pairs P] [--mac-epochs E]
+                    [--n N] [--leaf L] [--degree D]
+
+The CPU baseline is the oracle on the host's cores (at most 16), on every
+target; above 1M points it samples pairs and passes in blocks of 64 and
+scales each part by the fraction of its work done.
 """
 from __future__ import annotations
 
