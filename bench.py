@@ -21,9 +21,10 @@ sampled targets above).  Launch shapes are autotuned before the warmup.
                     [--mac-budget B] [--mac-sets S] [--mac-pairs P] [--mac-epochs E]
                     [--n N] [--leaf L] [--degree D]
 
-The CPU baseline is the oracle on the host's cores (at most 16), on every
-target; above 1M points it samples pairs and passes in blocks of 64 and
-scales each part by the fraction of its work done.
+The CPU baseline is the oracle on the host's cores (at most 16) on the same
+problem: up to 1M points it runs every 10th block of 64 PP/PC pairs, above
+also a sample of the CP/CC pairs and of the passes' points, panels and leaves,
+and scales each part by the fraction of its work done.
 """
 from __future__ import annotations
 
