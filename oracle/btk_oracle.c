@@ -415,6 +415,14 @@ void orc_panel_points_inside(const orc_tree *t, int thresh, int *out) {
 /* traversal: interaction_list.cpp:5-278                                      */
 /* ------------------------------------------------------------------------ */
 
+/* Children are pushed c1..c8 (:204-238).  ORC_PLANT_PUSH_ORDER is a planted
+ * mistake built only by the list tests: the last two children swap places. */
+#ifdef ORC_PLANT_PUSH_ORDER
+#define ORC_PUSH_CHILD(c, nc) ((c) >= (nc) - 2 ? (2 * (nc) - 3 - (c)) : (c))
+#else
+#define ORC_PUSH_CHILD(c, nc) (c)
+#endif
+
 int orc_dual_tree_traversal(orc_run_config *cfg, const orc_tree *tt, const orc_tree *ts, orc_pair **list_out) {
     const int dim = tt->dim, nc = dim == 3 ? 8 : 4;
     const int thr = cfg->fmm_cluster_thresh;
@@ -460,7 +468,11 @@ int orc_dual_tree_traversal(orc_run_config *cfg, const orc_tree *tt, const orc_t
                 refine_s = 1;
             else if (S->is_leaf)
                 refine_t = 1;
+#ifdef ORC_PLANT_REFINE_GE /* planted mistake, built only by the list tests: '>=' for '>' */
+            else if (T->point_count >= S->point_count)
+#else
             else if (T->point_count > S->point_count)
+#endif
                 refine_t = 1;
             else
                 refine_s = 1;
@@ -481,12 +493,12 @@ int orc_dual_tree_traversal(orc_run_config *cfg, const orc_tree *tt, const orc_t
         } else if (refine_t) {
             for (int c = 0; c < nc; c++) {
                 ivec_push(&qs, is);
-                ivec_push(&qt, T->child[c]);
+                ivec_push(&qt, T->child[ORC_PUSH_CHILD(c, nc)]);
             }
         } else if (refine_s) {
             for (int c = 0; c < nc; c++) {
                 ivec_push(&qt, it);
-                ivec_push(&qs, S->child[c]);
+                ivec_push(&qs, S->child[ORC_PUSH_CHILD(c, nc)]);
             }
         }
     }

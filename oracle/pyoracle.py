@@ -86,42 +86,56 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        dp = C.POINTER(C.c_double)
-        ip = C.POINTER(C.c_int)
-        L.orc_read_run_config.argtypes = [C.c_char_p, C.POINTER(RunConfig)]
-        L.orc_interp_vals_bli.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int]
-        L.orc_bli_points_shift.argtypes = [dp, C.c_double, C.c_double, C.c_int]
-        L.orc_tree_build.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, C.POINTER(Tree)]
-        L.orc_tree_free.argtypes = [C.POINTER(Tree)]
-        L.orc_pack_panels_2d.argtypes = [C.POINTER(Tree), C.c_void_p]
-        L.orc_pack_panels_3d.argtypes = [C.POINTER(Tree), C.c_void_p]
-        L.orc_panel_points_inside.argtypes = [C.POINTER(Tree), C.c_int, ip]
-        L.orc_dual_tree_traversal.argtypes = [C.POINTER(RunConfig), C.POINTER(Tree), C.POINTER(Tree),
-                                              C.POINTER(C.POINTER(Pair))]
-        L.orc_split_interactions.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
-        L.orc_upward_pass.argtypes = [C.POINTER(RunConfig), C.POINTER(Tree), dp, dp, dp, dp, dp]
-        L.orc_fmm_interactions.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.POINTER(Tree),
-                                           C.POINTER(Tree), dp, dp, dp, dp, dp, dp, dp, dp, C.c_void_p,
-                                           C.c_int, dp, dp]
-        L.orc_downward_pass.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.POINTER(Tree), dp, dp,
-                                        dp, dp, dp]
-        L.orc_direct_sum.argtypes = [C.c_int, C.c_double, C.POINTER(Opts), C.c_int, ip, dp, dp, dp,
-                                     C.c_int, dp, dp, dp, dp, dp]
-        L.orc_solve.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.c_int, dp, dp, dp, C.c_int, dp,
-                                dp, dp, dp, C.c_int, dp, dp, ip]
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_get_threads.restype = C.c_int
-        L.orc_components.argtypes = [C.c_int, C.c_int]
-        L.orc_last_near_seconds.restype = C.c_double
-        L.orc_last_near_fraction.restype = C.c_double
-        L.orc_last_far_seconds.restype = C.c_double
-        L.orc_last_far_fraction.restype = C.c_double
-        L.orc_last_passes.argtypes = [dp, dp]
-        L.free = C.CDLL(None).free
-        L.free.argtypes = [C.c_void_p]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def use_library(path: str | None):
+    """Switches every function of this module to the restatement built at
+    `path` (None: the default build); returns the previous library.  The
+    list tests use it to load builds with a planted mistake."""
+    global _lib
+    prev = _lib
+    _lib = None if path is None else _load(path)
+    return prev
+
+
+def _load(path: str):
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.orc_read_run_config.argtypes = [C.c_char_p, C.POINTER(RunConfig)]
+    L.orc_interp_vals_bli.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int]
+    L.orc_bli_points_shift.argtypes = [dp, C.c_double, C.c_double, C.c_int]
+    L.orc_tree_build.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, C.POINTER(Tree)]
+    L.orc_tree_free.argtypes = [C.POINTER(Tree)]
+    L.orc_pack_panels_2d.argtypes = [C.POINTER(Tree), C.c_void_p]
+    L.orc_pack_panels_3d.argtypes = [C.POINTER(Tree), C.c_void_p]
+    L.orc_panel_points_inside.argtypes = [C.POINTER(Tree), C.c_int, ip]
+    L.orc_dual_tree_traversal.argtypes = [C.POINTER(RunConfig), C.POINTER(Tree), C.POINTER(Tree),
+                                          C.POINTER(C.POINTER(Pair))]
+    L.orc_split_interactions.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+    L.orc_upward_pass.argtypes = [C.POINTER(RunConfig), C.POINTER(Tree), dp, dp, dp, dp, dp]
+    L.orc_fmm_interactions.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.POINTER(Tree),
+                                       C.POINTER(Tree), dp, dp, dp, dp, dp, dp, dp, dp, C.c_void_p,
+                                       C.c_int, dp, dp]
+    L.orc_downward_pass.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.POINTER(Tree), dp, dp,
+                                    dp, dp, dp]
+    L.orc_direct_sum.argtypes = [C.c_int, C.c_double, C.POINTER(Opts), C.c_int, ip, dp, dp, dp,
+                                 C.c_int, dp, dp, dp, dp, dp]
+    L.orc_solve.argtypes = [C.POINTER(RunConfig), C.POINTER(Opts), C.c_int, dp, dp, dp, C.c_int, dp,
+                            dp, dp, dp, C.c_int, dp, dp, ip]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
+    L.orc_components.argtypes = [C.c_int, C.c_int]
+    L.orc_last_near_seconds.restype = C.c_double
+    L.orc_last_near_fraction.restype = C.c_double
+    L.orc_last_far_seconds.restype = C.c_double
+    L.orc_last_far_fraction.restype = C.c_double
+    L.orc_last_passes.argtypes = [dp, dp]
+    L.free = C.CDLL(None).free
+    L.free.argtypes = [C.c_void_p]
+    return L
 
 
 def _dp(a):

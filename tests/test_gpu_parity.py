@@ -210,3 +210,27 @@ def test_mixed_precision_close_to_fp64(kind, dim, n, deg, thresh, theta, eps, ke
     print(f"  mixed vs fp64 {e:.2e}; rel-L2 vs direct sum: fp64 {l64:.2e} mixed {l32:.2e}", flush=True)
     assert e < MIXED_TOL
     assert l32 < l64 + MIXED_TOL
+
+
+def test_lists_equal_hand_derived():
+    """The GPU traversal against the hand-derived lists of tests/golden/tiny_lists.json."""
+    import json
+    import os
+    here = os.path.dirname(__file__)
+    doc = json.load(open(os.path.join(here, "golden", "tiny_lists.json")))
+    trees = {t["name"]: t for t in json.load(open(os.path.join(here, "golden", "tiny_trees.json")))["trees"]}
+    trees.update(doc["sources"])
+    for case in doc["cases"]:
+        print(f"\n[hand list] {case['name']}", flush=True)
+        tdef = trees[case["tree"]]
+        dim, thresh = tdef["dim"], tdef["thresh"]
+        cfg = btk.make_config(dim, 2, thresh, case["theta"], 0.0)
+
+        def tree(d):
+            return btk.Tree(cfg, dev(np.array(d["x"])), dev(np.array(d["y"])),
+                            dev(np.array(d["z"])) if dim == 3 else None)
+        tt = tree(tdef)
+        ts = tree(trees[case["source"]]) if "source" in case else tt
+        pairs = btk.InteractionList(cfg, tt, ts).pairs()
+        got = [[int(e["target_panel"]), int(e["source_panel"]), int(e["interact_type"])] for e in pairs]
+        assert got == case["list"], case["name"]
