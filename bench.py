@@ -1,30 +1,40 @@
-"""Benchmark of the MI355X treecode.  Default workload: 2D Poisson potential+gradient.
+"""Benchmark of the MI355X treecode.  Default workload: 16M-point 3D Plummer potential+gradient.
 
 Workload (one step = one full solve: tree, traversal, upward, interactions,
 downward, on inputs already resident in HBM):
-    N = M = 1,000,000 coincident points in 2D: 800k from 4 isotropic Gaussians
-    (sigma 0.05, centres U[-1,1]^2) + 200k uniform in [-1,1]^2; charges N(0,1);
-    kernel -log r / (2 pi) and its gradient; interpolation degree 6, theta 0.6,
-    leaf size 1000; eps = 0 with zero-distance (self) pairs excluded.
+    N = M = 16,000,000 coincident points, Plummer sphere a = 0.3 truncated at
+    r <= 10; charges U[-1,1]; kernel 1/(4 pi r) and its gradient; degree 8,
+    theta 0.7 (the fixed MAC and the learned MAC's fallback), leaf 1300;
+    eps = 0 with zero-distance (self) pairs excluded.
+
+Three variants are timed on it: the plain FP64 treecode, the mixed-precision
+treecode with the fixed theta MAC, and the mixed-precision treecode with the
+learned MAC.  The learned MAC's network is trained first on cluster pairs
+sampled from the treecode's own runs on other Plummer sets, and its per-pair
+error budget is calibrated on held-out Plummer sets of the same size (other
+seeds) against their FP64 direct sums; the benchmarked input's own direct sum
+only judges the result afterwards.
 
 Prints one JSON line.  `value` is targets/s of the whole job for the fastest
-variant that meets the metric's 1e-6 error (mixed precision, or in 3D the
-learned MAC; mixed precision when neither does).  The accuracy of every variant is its relative
-L2 error against an FP64 direct sum on the GPU (every target up to 1M, 10k
-sampled targets above).  Launch shapes are autotuned before the warmup.
+variant whose relative L2 error against the FP64 direct sum (every target up
+to 1M, 10k sampled targets above) is at most 1e-6.  When no variant meets it,
+the same workload is re-run at the smallest degree that does, and that run is
+the value; if none does, `value` is null.  Launch shapes are autotuned before
+the warmup.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant both|plain|mixed|learned]
-                    [--workload 2d|3d-uniform-100k|3d-sphere-plummer-4M|3d-plummer-16M|
+                    [--workload 3d-plummer-16M|2d|3d-uniform-100k|3d-sphere-plummer-4M|
                                 3d-batched-2M|3d-forest-256]
                     [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
                     [--no-cpu-baseline] [--cpu-stride K] [--no-accuracy] [--no-degree-scan]
-                    [--mac-budget B] [--mac-sets S] [--mac-pairs P] [--mac-epochs E]
-                    [--n N] [--leaf L] [--degree D]
+                    [--mac-budget B] [--mac-target E] [--mac-holdout H]
+                    [--mac-sets S] [--mac-pairs P] [--mac-epochs E] [--n N] [--leaf L] [--degree D]
 
-The CPU baseline is the oracle on the host's cores (at most 16) on the same
-problem: up to 1M points it runs every 10th block of 64 PP/PC pairs, above
-also a sample of the CP/CC pairs and of the passes' points, panels and leaves,
-and scales each part by the fraction of its work done.
+The CPU baseline is the oracle (the C/OpenMP restatement of the reference) on
+the host threads OMP_NUM_THREADS names (all the process may use otherwise), on
+the same problem: up to 1M points it runs every 10th block of 64 PP/PC pairs,
+above also a sample of the CP/CC pairs and of the passes' points, panels and
+leaves, and scales each part by the fraction of its work done.
 """
 from __future__ import annotations
 
@@ -43,6 +53,12 @@ SEED = 20261203
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # flops per kernel evaluation (FMA = 2, transcendental = 1)
 FLOPS = {(2, 0): 9, (2, 2): 15, (3, 0): 12, (3, 1): 17, (3, 2): 20}
+FLOPS_NOTE = ("algorithmic flops: (E_PP + E_PC) evaluations x flops per evaluation, FMA = 2 and a log, square "
+              "root, reciprocal or division = 1 (3D potential+gradient 20, 2D 15); the kernels run on the vector "
+              "ALU and its transcendental unit, whose FP32 log2/rcp/rsqrt issue at a quarter of the FMA rate, and "
+              "the FP64 log of the plain 2D kernel is a software routine of tens of instructions; the peak is the "
+              "published vector peak of the arithmetic type (FP32 counts packed FMA, which these kernels do not "
+              "use)")
 
 # name: (dim, n, generator, degree, theta, leaf, kernel, description)
 WORKLOADS = {
@@ -86,7 +102,8 @@ def parse():
     ap.add_argument("--variant", choices=["both", "plain", "mixed", "learned"], default="both",
                     help="both = plain and mixed, and in 3D also mixed with the learned MAC")
     ap.add_argument("--no-degree-scan", action="store_true")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST), default="2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(FOREST),
+                    default="3d-plummer-16M")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--leaf", type=int, default=None, help="leaf size (cluster_size) instead of the workload's")
     ap.add_argument("--degree", type=int, default=None, help="interpolation degree instead of the workload's")
@@ -97,7 +114,10 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--mac-budget", type=float, default=None,
-                    help="learned MAC: this per-pair budget instead of the scan (for profiling)")
+                    help="learned MAC: this per-pair budget instead of the held-out calibration (for profiling)")
+    ap.add_argument("--mac-target", type=float, default=0.85e-6,
+                    help="learned MAC: rel-L2 the calibration asks of every held-out set (margin below 1e-6)")
+    ap.add_argument("--mac-holdout", type=int, default=2, help="learned MAC: held-out sets for the calibration")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline up to 1M points: every k-th block of 64 PP/PC pairs")
@@ -106,9 +126,99 @@ def parse():
     return ap.parse_args()
 
 
-# per-pair error budgets of the learned MAC, tried from the largest down
-MAC_BUDGETS = (2e-3, 1e-3, 6e-4, 4e-4, 3e-4, 2e-4, 1.5e-4, 1e-4, 8e-5, 7e-5, 6e-5, 5.5e-5, 5e-5, 4.5e-5, 4e-5,
-               3.5e-5, 3e-5, 2e-5, 1e-5, 5e-6, 2e-6)
+# per-pair error budgets of the learned MAC the calibration chooses from:
+# 2e-3 down to 2e-6 in steps of 10^(1/12) (about 21 %)
+MAC_BUDGETS = tuple(2e-3 * 10 ** (-k / 12) for k in range(37))
+
+
+def make_points(gen, n, seed):
+    from barytreek_amd import workloads as wl
+    if gen == "gauss2":
+        return wl.gaussian_mixture_2d(n, seed)
+    if gen == "cube3":
+        return wl.uniform_cube(n, 3, seed)
+    if gen == "plummer3":
+        return wl.plummer_3d(n, seed, a=0.3, rmax=10.0)
+    return wl.sphere_plummer_3d(n, seed)
+
+
+def sample_idx(n, seed, torch, dev):
+    """Targets the direct sum checks: every one up to 1M, else 10k sampled."""
+    if n <= 1_000_000:
+        return None
+    g = np.random.default_rng(seed)
+    return torch.from_numpy(np.sort(g.choice(n, 10_000, replace=False)).astype(np.int32)).to(dev)
+
+
+def direct_sum(btk, torch, dim, kernel, C, d, idx):
+    n = d[0].numel()
+    m = n if idx is None else int(idx.numel())
+    ds = torch.zeros(C * m, dtype=torch.float64, device=d[0].device)
+    btk.direct_sum(dim, btk.options(kernel=kernel, convention=1, exclude_self=1), 0.0, d[0], d[1], d[2],
+                   d[0], d[1], d[2], d[3], ds, idx=idx)
+    torch.cuda.synchronize()
+    return ds.view(C, m)
+
+
+def rel_l2(out, ds, idx, n):
+    o = out.view(-1, n)
+    if idx is not None:
+        o = o[:, idx.long()]
+    o = o.reshape(ds.shape)
+    return float((o - ds).norm() / ds.norm())
+
+
+def calibrate_budget(btk, torch, spec, n, cfg, opts, target, holdout, dev):
+    """Per-pair budget of the learned MAC for this workload, fixed before the
+    benchmarked input is seen: on `holdout` point sets of the same generator
+    and size (other seeds), the largest budget of MAC_BUDGETS whose solve
+    stays within `target` rel-L2 of that set's FP64 direct sum.  Bisection on
+    the first set (the error falls as the budget falls), then a step down
+    while any other set misses the target."""
+    dim, _, gen, _, _, _, kernel, _ = spec
+    C = btk.components(dim, kernel)
+    sets, trials = [], []
+    for h in range(holdout):
+        seed = SEED + 1000 + h
+        pts = make_points(gen, n, seed)
+        d = [None if a is None else torch.from_numpy(a).to(dev) for a in pts]
+        idx = sample_idx(n, seed + 1, torch, dev)
+        sets.append((seed, d, idx, direct_sum(btk, torch, dim, kernel, C, d, idx)))
+    out = torch.zeros(C * n, dtype=torch.float64, device=dev)
+
+    def err(k, si):
+        seed, d, idx, ds = sets[si]
+        btk.mac_set_budget(MAC_BUDGETS[k])
+        out.zero_()
+        btk.solve(cfg, opts, d[0], d[1], d[2], d[3], out)
+        e = rel_l2(out, ds, idx, n)
+        trials.append({"set_seed": seed, "budget": MAC_BUDGETS[k], "rel_l2": e})
+        log(f"calibration: set {seed} budget {MAC_BUDGETS[k]:.3g} rel-L2 {e:.3g}")
+        return e
+
+    lo, hi = 0, len(MAC_BUDGETS) - 1       # lo: largest budget; hi: smallest
+    if err(hi, 0) > target:
+        k = hi
+    elif err(lo, 0) <= target:
+        k = lo
+    else:
+        while hi - lo > 1:                 # err(lo) > target >= err(hi)
+            mid = (lo + hi) // 2
+            if err(mid, 0) <= target:
+                hi = mid
+            else:
+                lo = mid
+        k = hi
+    for si in range(1, len(sets)):
+        while k < len(MAC_BUDGETS) - 1 and err(k, si) > target:
+            k += 1
+    del sets, out
+    torch.cuda.empty_cache()
+    return MAC_BUDGETS[k], {
+        "method": (f"largest budget (grid 2e-3 * 10^(-k/12)) whose solve of each of {holdout} held-out sets "
+                   f"(same generator and size, seeds {SEED + 1000}..{SEED + 999 + holdout}) is within "
+                   f"{target:g} rel-L2 of that set's FP64 direct sum; the benchmarked input is not used"),
+        "target": target, "trials": trials}
 
 
 def train_mac(btk, torch, wl, degree, leaf, theta, sets=10, per=100_000, epochs=20):
@@ -153,6 +263,23 @@ def cpu_strides(n, near_stride):
     return max(1, n // 20_000), max(1, n // 20_000), max(1, n // 250_000)
 
 
+def host_cpu():
+    """The host's CPU as the baseline sees it: model, nproc, the cores this
+    process may run on, and OMP_NUM_THREADS (the thread count used when set)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": int(omp) if omp and omp.isdigit() else None}
+
+
 def cpu_baseline(wl_spec, x, y, z, q, near_stride, far_stride=1, pass_stride=1):
     """The CPU restatement (oracle) of the same workload, OpenMP on the host's
     cores.  Every stride-th block of 64 consecutive units is run (PP/PC pairs,
@@ -161,8 +288,8 @@ def cpu_baseline(wl_spec, x, y, z, q, near_stride, far_stride=1, pass_stride=1):
     (evaluations for the pairs, units for the passes).  Tree and traversal run
     in full."""
     from oracle import pyoracle as O
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
+    host = host_cpu()
+    cores = host["omp_num_threads"] or host["affinity"]
     O.set_threads(cores)
     dim, _, _, deg, theta, leaf, kernel, _ = wl_spec
     cfg = O.make_config(dim, deg, leaf, theta, 0.0)
@@ -179,6 +306,7 @@ def cpu_baseline(wl_spec, x, y, z, q, near_stride, far_stride=1, pass_stride=1):
         "value": len(x) / est,
         "unit": "targets/s",
         "cores": cores,
+        "host": host,
         "kind": "port",
         "sample": (f"oracle (C/OpenMP restatement) on the same {len(x)}-point problem: tree and traversal in full; "
                    f"every {near_stride}th block of 64 PP/PC pairs ({near_f:.4f} of their evaluations, "
@@ -222,9 +350,10 @@ def summarize(stats, ms, n, dtype, out, ds, flops_per_eval, idx=None):
         "phases_s": {k: avg("t_" + k) for k in ("tree", "traversal", "upward", "interact", "downward", "total")},
         "t_kernel_near_s": t_near,
         "t_kernel_far_s": avg("t_kernel_far"),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_TFLOPS[dtype], 4), "traffic": None,
-                     "kernel": "k_near (PP+PC)", "flops_per_launch": near_flops, "avg_launch_s": t_near},
+                     "kernel": "k_interact near (PP+PC launches of one solve)", "flops_per_launch": near_flops,
+                     "avg_launch_s": t_near, "flops_convention": FLOPS_NOTE},
         "evals": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
         "pairs": [int(stats[-1].count[k]) for k in range(4)],
         "cc_shared": int(stats[-1].cc_shared),
@@ -413,7 +542,7 @@ def bench_forest(args, btk, torch, wl, dev):
     t_near = avg(st_m, "t_kernel_near")
     if t_near > 0:
         ach = (evals[0] + evals[1]) * FLOPS[(3, 0)] / t_near / 1e12
-        line["roofline"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["f32"],
+        line["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["f32"],
                             "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS["f32"], 4), "traffic": None,
                             "kernel": "k_interact near (PP+PC), FP32", "avg_launch_s": t_near}
     print(json.dumps(line), flush=True)
@@ -453,14 +582,8 @@ def main():
         degree = args.degree
         spec = spec[:3] + (degree,) + spec[4:]
         desc = f"{desc} [degree {degree}]"
-    if gen == "gauss2":
-        x, y, z, q = wl.gaussian_mixture_2d(n, SEED)
-    elif gen == "cube3":
-        x, y, z, q = wl.uniform_cube(n, 3, SEED)
-    elif gen == "plummer3":
-        x, y, z, q = wl.plummer_3d(n, SEED, a=0.3, rmax=10.0)
-    else:
-        x, y, z, q = wl.sphere_plummer_3d(n, SEED)
+    log(f"workload {args.workload}: generating {n} points")
+    x, y, z, q = make_points(gen, n, SEED)
     d = [None if a is None else torch.from_numpy(a).to(dev) for a in (x, y, z, q)]
     C = btk.components(dim, kernel)
     fpe = FLOPS[(dim, kernel)]
@@ -474,109 +597,102 @@ def main():
     # target up to 1M, 10k sampled targets above
     ds, idx = None, None
     if not args.no_accuracy:
-        if n > 1_000_000:
-            g = np.random.default_rng(SEED + 1)
-            idx = torch.from_numpy(np.sort(g.choice(n, 10_000, replace=False)).astype(np.int32)).to(dev)
-        m = n if idx is None else int(idx.numel())
-        ds = torch.zeros(C * m, dtype=torch.float64, device=dev)
-        log(f"FP64 direct sum on {m} targets")
-        btk.direct_sum(dim, btk.options(kernel=kernel, convention=1, exclude_self=1), 0.0, d[0], d[1], d[2],
-                       d[0], d[1], d[2], d[3], ds, idx=idx)
-        torch.cuda.synchronize()
-        ds = ds.view(C, m)
+        idx = sample_idx(n, SEED + 1, torch, dev)
+        log(f"FP64 direct sum on {n if idx is None else int(idx.numel())} targets")
+        ds = direct_sum(btk, torch, dim, kernel, C, d, idx)
 
     res = {}
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
         opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"))
-        train = None
+        train = choice = None
+        budget = None
         if v == "learned":  # untimed, like the warmup
             log("learned MAC: samples and training")
             train = train_mac(btk, torch, wl, degree, leaf, theta, args.mac_sets, args.mac_pairs, args.mac_epochs)
+            btk.set_tuning()
+            if args.mac_budget:
+                budget, choice = args.mac_budget, {"method": "given on the command line"}
+            else:
+                log("learned MAC: budget calibration on held-out sets")
+                budget, choice = calibrate_budget(btk, torch, spec, n, cfg, opts, args.mac_target, args.mac_holdout,
+                                                  dev)
+            btk.mac_set_budget(budget)
         btk.set_tuning()  # defaults
         if args.shapes:
             btk.set_tuning(*[int(x) for x in args.shapes.split(",")])
         elif not args.no_autotune:  # untimed, like the warmup
             log(f"autotune {v}")
-            if v == "learned":
-                btk.mac_set_budget(MAC_BUDGETS[0])
             btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
         t = btk.get_tuning()
+        log(f"timing {v}")
+        ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
+        res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
+        del out
         if v == "learned":
-            # the largest per-pair budget whose result stays within 1e-6
-            scan = []
-            for b in ([args.mac_budget] if args.mac_budget else MAC_BUDGETS):
-                btk.mac_set_budget(b)
-                log(f"timing learned, budget {b:g}")
-                ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
-                r = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
-                del out
-                scan.append({"budget": b, "ms_per_step": ms, "rel_l2": r.get("rel_l2")})
-                if ds is None or r["rel_l2"] <= 1e-6:
-                    break
-            res[v] = dict(r, mac_budget=b, mac_budget_scan=scan, mac_training=train)
-        else:
-            log(f"timing {v}")
-            ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
-            res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
-            del out
+            res[v].update(mac_budget=budget, mac_budget_choice=choice, mac_training=train)
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
                                    "far_tpt": t.far_tpt, "mutual": t.mutual,
                                    "autotuned": not (args.no_autotune or args.shapes)}
-    head = "mixed" if "mixed" in res else variants[0]
+        log(f"{v}: {ms:.2f} ms/solve, rel-L2 {res[v].get('rel_l2')}")
     # the fastest variant that meets the metric's error
-    meets = [v for v in ("mixed", "learned") if v in res and (ds is None or res[v]["rel_l2"] <= 1e-6)]
-    if meets:
-        head = min(meets, key=lambda v: res[v]["ms_per_step"])
-    main_r = res[head]
+    meets = [v for v in res if ds is None or res[v]["rel_l2"] <= 1e-6]
+    head = min(meets, key=lambda v: res[v]["ms_per_step"]) if meets else None
 
-    # the same workload at the smallest degree whose error reaches the metric's 1e-6
+    # no variant meets it: the same workload at the smallest degree that does
     at = None
-    if ds is not None and not args.no_degree_scan:
-        for deg in range(degree, 11):
+    if head is None and ds is not None and not args.no_degree_scan:
+        v = "learned" if "learned" in res else ("mixed" if "mixed" in res else variants[0])
+        for deg in range(degree + 1, 11):
             log(f"degree scan: degree {deg}")
             cfg = btk.make_config(dim, deg, leaf, theta, 0.0)
-            opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[head],
-                               mac=int(head == "learned"))
-            ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, 1)
-            r = summarize(stats, ms, n, dtype[head], out, ds, fpe, idx)
+            opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v],
+                               mac=int(v == "learned"))
+            ms, out, stats = run_variant(btk, torch, cfg, opts, d, C, n, args.steps, args.warmup)
+            r = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
             del out
             if r["rel_l2"] <= 1e-6:
-                at = {"degree": deg, "targets_per_s": r["targets_per_s"], "ms_per_step": ms, "rel_l2": r["rel_l2"],
-                      "variant": head}
+                at = dict(r, degree=deg, variant=v)
                 break
+    main_r = res[head] if head else at
+    if head is None and at is not None:
+        head = at["variant"]
+        desc = f"{desc} [run at degree {at['degree']}, the smallest that meets 1e-6]"
 
     line = {
         "metric": "targets/s at <=1e-6 rel-L2 vs FP64 direct sum; plain vs mixed/batched/learned",
-        "value": main_r["targets_per_s"],
+        "value": main_r["targets_per_s"] if main_r else None,
         "unit": "targets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": main_r["ms_per_step"],
+        "ms_per_step": main_r["ms_per_step"] if main_r else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": dtype[head],
+        "dtype": dtype[head] if head else None,
         "data": "synthetic",
         "config": {"workload": desc, "name": args.workload,
                    "variant": head, "n": n, "seed": SEED, "parallelism": "single GPU",
                    "accumulation": "f64 per target"},
-        "rel_l2": main_r.get("rel_l2"),
-        "rel_l2_potential": main_r.get("rel_l2_potential"),
-        "interactions_per_s": main_r["interactions_per_s"],
-        "roofline": main_r["roofline"],
+        "rel_l2": main_r.get("rel_l2") if main_r else None,
+        "rel_l2_potential": main_r.get("rel_l2_potential") if main_r else None,
+        "rel_l2_targets": main_r.get("rel_l2_targets") if main_r else None,
+        "interactions_per_s": main_r["interactions_per_s"] if main_r else None,
+        "roofline": main_r["roofline"] if main_r else None,
         "variants": res,
-        "at_rel_l2_1e-6": at,
+        "at_smallest_degree_meeting_1e-6": at,
         "cpu_baseline": None,
     }
+    if main_r is None:
+        line["metric_missed"] = "no variant or degree reached 1e-6 rel-L2; value is null"
     # HBM bytes per launch of the near kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size) of the
     # same kernel shape (mutual pairs or not, targets per lane)
-    sh = main_r.get("launch_shapes", {})
+    sh = main_r.get("launch_shapes", {}) if main_r else {}
     tfile = os.path.join(HERE, "profiles", f"traffic_k_interact_near_{args.workload}_{head}"
                                            f"_m{sh.get('mutual', 0)}_t{sh.get('near_tpt', 0) or (2 if dim == 2 else 1)}.json")
-    if os.path.exists(tfile):
+    if main_r and os.path.exists(tfile):
         t = json.load(open(tfile))
         line["roofline"] = dict(line["roofline"], traffic=t["traffic_bytes_per_launch"],
                                 traffic_source=os.path.relpath(tfile, HERE))
@@ -587,7 +703,8 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(spec, x, y, z, q, *cpu_strides(n, args.cpu_stride))
-        line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        if line["value"]:
+            line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
         if "plain" in res:
             line["plain_over_cpu"] = res["plain"]["targets_per_s"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
