@@ -105,6 +105,37 @@ def test_learned_degrees_match_oracle_degree_by_degree(kernel):
         assert np.array_equal(part, pairs[ty == k])
 
 
+@pytest.mark.parametrize("cc_group", [0, 2, -1])
+def test_learned_mixed_precision_vs_oracle_degree_by_degree(cc_group):
+    """Mixed precision on a learned list, with its CC pairs as shared-kernel-
+    matrix MFMA products (cc_group 0: keys of >= 8 pairs, 2: >= 2) or pair by
+    pair (-1), against the oracle run degree by degree in FP64: within 1e-6."""
+    print(f"\n[mac degrees] mixed, cc_group={cc_group}, vs oracle", flush=True)
+    old = btk.get_tuning()
+    x, y, z, q = _problem(8)
+    d = [torch.from_numpy(a).cuda() for a in (x, y, z, q)]
+    cfg = btk.make_config(3, DEG, THR, THETA, 0.0)
+    tree = btk.Tree(cfg, d[0], d[1], d[2])
+    pairs = btk.InteractionList(cfg, tree, tree, learned_mac=True).pairs()
+    C = btk.components(3, btk.KERNEL_POT_GRAD)
+    out = torch.zeros(C * N, dtype=torch.float64, device="cuda")
+    btk.set_tuning(cc_group=cc_group)
+    try:
+        st = btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, mac=1, precision=btk.PREC_MIXED),
+                       d[0], d[1], d[2], d[3], out)
+    finally:
+        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual, old.cc_group)
+    print(f"  CC pairs {int(st.count[3])}, as shared-matrix products {int(st.cc_shared)}", flush=True)
+    if cc_group == -1:
+        assert st.cc_shared == 0
+    elif cc_group == 2:
+        assert st.cc_shared > 0
+    ref = _oracle_by_degree(pairs, btk.KERNEL_POT_GRAD, x, y, z, q)
+    rel = np.linalg.norm(out.view(C, N).cpu().numpy() - ref) / np.linalg.norm(ref)
+    print(f"  rel vs oracle {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+
+
 def test_learned_plan_and_batch_match_the_solve():
     print("\n[mac degrees] plan and batch vs solve", flush=True)
     x, y, z, q = _problem(6)

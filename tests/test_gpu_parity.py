@@ -99,6 +99,9 @@ STAGE_CASES = [
     ("gauss2", 2, 30000, 6, 300, 0.6, 0.0, btk.KERNEL_POT_GRAD, 1),
     ("cube2", 2, 20000, 8, 200, 0.6, 1e-4, btk.KERNEL_POT, 0),
     ("cube3", 3, 6000, 10, 300, 0.7, 1e-4, btk.KERNEL_POT, 0),
+    # sphere + Plummer potential+gradient at the degrees 8 and 10
+    ("sphere3", 3, 24000, 8, 400, 0.7, 0.0, btk.KERNEL_POT_GRAD, 1),
+    ("sphere3", 3, 20000, 10, 600, 0.7, 0.0, btk.KERNEL_POT_GRAD, 1),
 ]
 
 
