@@ -33,6 +33,7 @@ EXPORTED = [
     "btk_dual_tree_traversal_2d", "btk_dual_tree_traversal_3d", "btk_dual_tree_traversal_learned_3d",
     "btk_list_count", "btk_list_copy",
     "btk_split_interactions", "btk_list_destroy", "btk_upward_pass_2d", "btk_upward_pass_3d",
+    "btk_upward_pass_3d_noalloc",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
     "btk_plan_create", "btk_plan_execute", "btk_plan_destroy",
@@ -138,6 +139,7 @@ def lib():
     L.btk_list_destroy.restype = None
     L.btk_upward_pass_2d.argtypes = [cfgp, vp, dp, dp, vp]
     L.btk_upward_pass_3d.argtypes = [cfgp, vp, dp, dp, vp]
+    L.btk_upward_pass_3d_noalloc.argtypes = [cfgp, vp, dp, dp, vp]
     L.btk_fmm_interactions_2d.argtypes = [cfgp, optp, vp, vp, vp, dp, dp, dp, dp, vp]
     L.btk_fmm_interactions_3d.argtypes = [cfgp, optp, vp, vp, vp, dp, dp, dp, dp, vp]
     L.btk_downward_pass_2d.argtypes = [cfgp, optp, vp, dp, dp, vp]
@@ -351,6 +353,16 @@ def upward_pass_2d(cfg, tree, charges, W, stream=None):
 
 
 upward_pass_3d = upward_pass_2d
+
+NOALLOC_MAX_POINTS = 8388608  # BTK_NOALLOC_MAX_POINTS
+
+
+def upward_pass_3d_noalloc(cfg, tree, charges, W, stream=None):
+    """btk_upward_pass_3d_noalloc: the upward pass with no device allocation
+    (its scratch is a fixed array per GPU); the work stays queued on the
+    stream."""
+    _check(lib().btk_upward_pass_3d_noalloc(C.byref(cfg), tree.h, _ptr(charges), _ptr(W), _stream(stream)))
+    return W
 
 
 def _interactions(cfg, opts, target, source, lst, charges, W, soln, Psi, stream=None):

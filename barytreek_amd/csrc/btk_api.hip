@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 
 #include "btk_internal.hpp"
@@ -406,6 +407,49 @@ static int upward(const btk_run_config *cfg, int dim, const btk_tree *source, co
         btk::gather_sorted(t, d_charges, q.p, s);
         btk::upward_pass(*cfg, t, q.p, d_W, s);
         BTK_HIP(hipStreamSynchronize(s));  // q is released on return
+    });
+}
+
+// Scratch of btk_upward_pass_3d_noalloc: the source charges in the tree's
+// sorted order.  Host code never uses the array's name as an address: each
+// device's copy is found once with hipGetSymbolAddress (with that device
+// current) and its pointer kept per device.
+static __device__ __align__(256) double scratch[BTK_NOALLOC_MAX_POINTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+double *scratch_on_current_device() {
+    int dev = 0;
+    BTK_HIP(hipGetDevice(&dev));
+    BTK_REQUIRE(dev >= 0 && dev < kMaxDevices, "device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        BTK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        BTK_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    return g_scratch[dev];
+}
+}  // namespace
+
+int btk_upward_pass_3d_noalloc(const btk_run_config *cfg, const btk_tree *source, const double *d_charges,
+                               double *d_W, void *stream) {
+    return guarded([&] {
+        check_cfg(cfg, 3);
+        BTK_REQUIRE(source && source->t->dim == 3, "bad source tree");
+        const btk::Tree &t = *source->t;
+        BTK_REQUIRE(t.n <= BTK_NOALLOC_MAX_POINTS,
+                    "btk_upward_pass_3d_noalloc: the source tree has " + std::to_string(t.n) +
+                        " points, more than its scratch holds (BTK_NOALLOC_MAX_POINTS = " +
+                        std::to_string(BTK_NOALLOC_MAX_POINTS) + ")");
+        double *q = scratch_on_current_device();
+        hipStream_t s = as_stream(stream);
+        btk::gather_sorted(t, d_charges, q, s);
+        btk::upward_pass(*cfg, t, q, d_W, s);
     });
 }
 

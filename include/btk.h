@@ -204,6 +204,22 @@ int btk_upward_pass_2d(const btk_run_config *cfg, const btk_tree *source, const 
 int btk_upward_pass_3d(const btk_run_config *cfg, const btk_tree *source, const double *d_charges,
                        double *d_proxy_source_weights, void *stream);
 
+/* No counterpart in the reference: btk_upward_pass_3d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating library routine: the charges in the tree's sorted order, the one
+ * scratch buffer of the upward pass, live in a fixed device array of
+ * BTK_NOALLOC_MAX_POINTS doubles (64 MiB) that each GPU the library runs on
+ * holds until the process ends.  A source tree of more points is refused
+ * with an error.  Same result as btk_upward_pass_3d.  Unlike it, the call
+ * does not synchronize: the work stays queued on `stream`.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (on different
+ * streams, or before an earlier call's work on another stream has finished)
+ * race; calls on one stream are ordered. */
+#define BTK_NOALLOC_MAX_POINTS 8388608
+int btk_upward_pass_3d_noalloc(const btk_run_config *cfg, const btk_tree *source, const double *d_charges,
+                               double *d_proxy_source_weights, void *stream);
+
 /* poisson_fmm_interactions_2d/_3d (poisson_fmm.hpp:9-15) and
  * poisson_vel_fmm_interactions_3d (poisson_vel_fmm.hpp:9-11), the kernel
  * chosen by opts.  Adds PP and PC to d_soln[C][M] (original target order)
