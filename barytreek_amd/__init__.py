@@ -69,7 +69,7 @@ class Options(C.Structure):
 class Tuning(C.Structure):
     """btk_tuning: launch shapes of the interaction kernels (0 = default)."""
     _fields_ = [("near_items", C.c_int), ("far_items", C.c_int), ("near_tpt", C.c_int), ("far_tpt", C.c_int),
-                ("mutual", C.c_int), ("cc_group", C.c_int), ("reserved", C.c_int * 2)]
+                ("mutual", C.c_int), ("cc_group", C.c_int), ("near_waves", C.c_int), ("far_waves", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -183,12 +183,18 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0, cc_group=0) -> None:
+def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0, cc_group=0, near_waves=0,
+               far_waves=0) -> None:
     """Process-wide launch shapes of the interaction kernels; 0 = default
     (cc_group: -1 turns the shared-matrix CC products off)."""
     t = Tuning()
     t.near_items, t.far_items, t.near_tpt, t.far_tpt = near_items, far_items, near_tpt, far_tpt
-    t.mutual, t.cc_group = mutual, cc_group
+    t.mutual, t.cc_group, t.near_waves, t.far_waves = mutual, cc_group, near_waves, far_waves
+    _check(lib().btk_set_tuning(C.byref(t)))
+
+
+def restore_tuning(t: Tuning) -> None:
+    """Installs a record from get_tuning() again."""
     _check(lib().btk_set_tuning(C.byref(t)))
 
 

@@ -143,6 +143,9 @@ int btk_set_tuning(const btk_tuning *t) {
                     "targets per lane must be 0 (default), 1 or 2");
         BTK_REQUIRE(t->mutual == 0 || t->mutual == 1, "mutual must be 0 or 1");
         BTK_REQUIRE(t->cc_group >= -1, "cc_group must be -1 (off), 0 (default) or a minimum group size");
+        auto waves_ok = [](int w) { return w == 0 || w == 1 || w == 2 || w == 4; };
+        BTK_REQUIRE(waves_ok(t->near_waves) && waves_ok(t->far_waves),
+                    "wavefronts per workgroup must be 0 (default), 1, 2 or 4");
         btk::tuning() = *t;
     });
 }
@@ -625,20 +628,24 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
             Psi(btk::proxy_offset(*l, tt.pc, C, l->ndeg)), out((size_t)C * (m ? m : 1));
         btk::gather_sorted(*ts, d_q, q.p, s);
         btk::upward_all(cfg, *l, *ts, q.p, W.p, s);
-        // Candidate shapes: work-item targets x targets per lane x mutual
-        // (the last two in mixed precision only).  The near and far launches
-        // are independent, so call i tries near shape i and far shape i.
+        // Candidate shapes: work-item targets x wavefronts per workgroup x
+        // targets per lane x mutual (the last two in mixed precision only).
+        // The near and far launches are independent, so call i tries near
+        // shape i and far shape i.
         const bool mixed = o.precision == BTK_PREC_MIXED;
         const long long items[] = {8192, 16384, 32768, 65536, 131072};
+        const int waves[] = {1, 2, 4};
         struct Shape {
-            int items, tpt, mutual;
+            int items, tpt, mutual, waves;
         };
         std::vector<Shape> nc, fc;
         for (int mu = 0; mu <= (mixed && coincident ? 1 : 0); mu++)
             for (int tpt = 1; tpt <= (mixed ? 2 : 1); tpt++)
-                for (long long it : items) nc.push_back({(int)it, tpt, mu});
+                for (int w : waves)
+                    for (long long it : items) nc.push_back({(int)it, tpt, mu, w});
         for (int tpt = 1; tpt <= (mixed ? 2 : 1); tpt++)
-            for (long long it : items) fc.push_back({(int)it, tpt, 0});
+            for (int w : waves)
+                for (long long it : items) fc.push_back({(int)it, tpt, 0, w});
         const int ncall = (int)std::max(nc.size(), fc.size());
         const btk_tuning saved = btk::tuning();
         btk::KernelEvents kev;
@@ -660,8 +667,11 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
                     t.near_items = a.items;
                     t.near_tpt = a.tpt;
                     t.mutual = a.mutual;
+                    t.near_waves = a.waves;
                     t.far_items = b.items;
                     t.far_tpt = b.tpt;
+                    t.far_waves = b.waves;
+                    t.cc_group = saved.cc_group;
                     btk::tuning() = t;
                     kev.near_done = kev.far_done = false;
                     const auto c0 = std::chrono::steady_clock::now();
@@ -697,11 +707,14 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
             best.near_items = nc[in].items;
             best.near_tpt = nc[in].tpt;
             best.mutual = nc[in].mutual;
+            best.near_waves = nc[in].waves;
         }
         if (tf[jf] < 1e30f) {
             best.far_items = fc[jf].items;
             best.far_tpt = fc[jf].tpt;
+            best.far_waves = fc[jf].waves;
         }
+        best.cc_group = saved.cc_group;
         if (o.precision != BTK_PREC_MIXED) best.near_tpt = best.far_tpt = 0;
         btk::tuning() = best;
         if (chosen) *chosen = best;

@@ -45,6 +45,25 @@ __global__ void k_entry_meta(int n, const int *__restrict__ gsrc, const unsigned
     meta[i] = (ty & 1) ? make_int4(S, 0, P, ty) : make_int4(S, start[S], count[S], ty);
 }
 
+// Mixed precision: flags the near entries (the first ndeg groups) whose boxes
+// are separated enough for relative FP32 coordinates.  The target panel of
+// entry e is found in goff (ascending).
+template <int DIM>
+__global__ void k_entry_rel(int n, int pct, int ndeg, const int *__restrict__ goff, TgtView tv, SrcView sv,
+                            int4 *__restrict__ meta) {
+    const int e = blockIdx.x * TB + threadIdx.x;
+    const int nk = ndeg * pct;
+    if (e >= n || e >= goff[nk]) return;
+    int lo = 0, hi = nk;  // goff[lo] <= e < goff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (goff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    const int T = lo % pct;
+    if (separated_for_rel<DIM>(T, meta[e].x, tv, sv)) meta[e].w |= kRelEntry;
+}
+
 // Entry slices balanced by source count (a leaf's points, or P proxies).
 // One wavefront per panel: the entries' source counts and their running sum
 // go to LDS, lane 0 places the slice boundaries, the lanes write the items.
@@ -161,6 +180,7 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.n_tasks = g.tk.n;
             L.n1 = n1;
             L.tpt = g.tpt;
+            L.waves = g.waves;
             L.far = far;
             L.excl = p.o.exclude_self ? 1 : 0;
             L.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
@@ -313,6 +333,17 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
         k_entry_meta<<<nblk(l.count), TB, 0, s>>>(l.count, l.gsrc.p, l.gtype.p, ts.start.p, ts.count.p,
                                                   make_int4(Pd[0], Pd[1], Pd[2], Pd[3]), p->meta.p);
         BTK_LAUNCH_CHECK();
+        if (o.precision == BTK_PREC_MIXED && tt.pc > 0) {
+            const TgtView tv{tt.start.p, tt.count.p, nullptr, tt.sx.p, tt.sy.p, tt.sz.p, tt.lo[0].p, tt.lo[1].p,
+                             tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
+            const SrcView sv{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, nullptr, ts.lo[0].p, ts.lo[1].p,
+                             ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, nullptr};
+            if (dim == 3)
+                k_entry_rel<3><<<nblk(l.count), TB, 0, s>>>(l.count, tt.pc, l.ndeg, l.goff.p, tv, sv, p->meta.p);
+            else
+                k_entry_rel<2><<<nblk(l.count), TB, 0, s>>>(l.count, tt.pc, l.ndeg, l.goff.p, tv, sv, p->meta.p);
+            BTK_LAUNCH_CHECK();
+        }
     }
     for (int far = 0; far < 2; far++) {
         if (l.counts[2 * far] + l.counts[2 * far + 1] == 0) continue;
@@ -323,7 +354,15 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
             // default: two targets per lane where the kernel is short (2D), one in 3D
             g.tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((dim == 3 || (far && P <= kLanes)) ? 1 : 2));
-            const int per = kLanes * g.tpt;
+            // default: the 3D near launch shares each source tile between 4
+            // wavefronts (a leaf holds hundreds of targets; 16M Plummer
+            // points: 3.5x fewer bytes fetched at the same time); in 2D the
+            // shorter kernel loses to the barriers (1M points: 9.4 against
+            // 8.6 ms), and the far launch's targets are the proxies of one
+            // cluster: one wavefront each
+            const int waves_set = far ? tu.far_waves : tu.near_waves;
+            g.waves = waves_set ? waves_set : ((far || dim == 2) ? 1 : 4);
+            const int per = kLanes * g.tpt * g.waves;
             // mixed precision, one tree, when asked for: mutual PP pairs once
             // for both directions (PP entries are in the degree-0 near group)
             g.mutual = !far && di == 0 && f32 && &tt == &ts && tu.mutual;

@@ -10,13 +10,15 @@
 //                       pairs (sources: a cluster's proxy points and weights);
 //   far  (FAR = true)   targets are the proxy points of a target cluster; the
 //                       entries are its CP and CC pairs.
-// A work item is one wavefront: up to 64*TPT targets of one panel and a
-// contiguous slice of the panel's entries.  Sources stream through a 64-entry
-// LDS tile (coalesced SoA loads of the tree-sorted points, or proxy points
-// generated from the source box) and every lane reads each tile entry as an
-// LDS broadcast.  When a panel's entries are split over several work items,
-// their partial sums meet through FP64 atomic adds; otherwise each target is
-// written once.
+// A work item is one workgroup of WAVES wavefronts (1, 2 or 4): up to
+// 64*TPT*WAVES targets of one panel, in consecutive blocks of 64*TPT per
+// wavefront, and a contiguous slice of the panel's entries.  Sources stream
+// through an LDS tile of 64*WAVES entries that the whole workgroup fills
+// (coalesced SoA loads of the tree-sorted points, or proxy points generated
+// from the source box), so each source is read from memory once per
+// workgroup; every lane reads each tile entry as an LDS broadcast.  When a
+// panel's entries are split over several work items, their partial sums meet
+// through FP64 atomic adds; otherwise each target is written once.
 //
 // Precision.  FP64: t - s and the kernel in FP64 (the reference's arithmetic).
 // Mixed (F32): t - s to FP32 relative precision (near: formed in FP64 and
@@ -126,10 +128,13 @@ struct TileMix<3> {
 
 // Per-entry source descriptor of a grouped list: x = source panel, y = first
 // sorted point of its points (0 for proxy sources), z = number of sources
-// (the leaf's point count, or P proxy points), w = interact_type.
+// (the leaf's point count, or P proxy points), w = interact_type (bits 0-3)
+// plus kRelEntry for a near entry whose boxes are separated enough for
+// relative FP32 coordinates (mixed precision; separated_for_rel).
 struct EntryView {
     const int4 *meta;
 };
+constexpr int kRelEntry = 16;
 
 
 // The proxy point idx of a box: nodes cos(pi/n*j) mapped to each axis
@@ -216,16 +221,19 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
 // The mixed-precision kernels ask for at least 6 wavefronts per SIMD (at most
 // 80 VGPRs); left to itself the compiler spends up to 113 on unrolling and
 // runs 4.  The FP64 kernels keep the compiler's choice.
-template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL>
-__global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
+template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES>
+__global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
     k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
     constexpr bool MIX = F32 && !FAR;                // FP64 or relative FP32 coordinates (TileMix)
+    constexpr int TS = kLanes * WAVES;               // sources per tile
     using Tile = std::conditional_t<REL, TileRel, std::conditional_t<MIX, TileMix<DIM>, TileSrc<F32>>>;
-    __shared__ Tile tile[kLanes];
-    const int lane = threadIdx.x;
+    __shared__ Tile tile[TS];
+    const int tid = threadIdx.x;
+    const int lane = tid % kLanes;
+    const int wave = tid / kLanes;
     const int item = xcd_contiguous(blockIdx.x, gridDim.x);
     const int T = tk.panel[item];
     const int first = tk.first[item];
@@ -233,6 +241,9 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
+    // this wavefront's targets: first + (wave * TPT + k) * 64 + lane
+    const int wfirst = first + wave * TPT * kLanes;
+    const bool active = wfirst < ntgt;  // uniform over the wavefront
     double t[TPT][3];
     float tr[TPT][3];
     double acc[TPT][C];
@@ -244,7 +255,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     }
 #pragma unroll
     for (int k = 0; k < TPT; k++) {
-        const int ti = min(first + lane + kLanes * k, ntgt - 1);
+        const int ti = min(wfirst + lane + kLanes * k, ntgt - 1);
         if constexpr (FAR) {
             // target proxy point (poisson_fmm_impl.hpp:352-361, :416-425)
             proxy_point<DIM>(ti, n1, tabs, tv.lo0[T], tv.hi0[T], tv.lo1[T], tv.hi1[T], DIM == 3 ? tv.lo2[T] : 0.0,
@@ -260,9 +271,10 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
         for (int c = 0; c < C; c++) acc[k][c] = 0.0;
     }
     // near, mixed: entries of separated boxes take relative coordinates
+    // (flagged when the plan built the entry descriptors)
     auto rel_entry = [&](const int4 &mm) {
         if constexpr (MIX)
-            return separated_for_rel<DIM>(T, mm.x, tv, sv);
+            return (mm.w & kRelEntry) != 0;
         else
             return false;
     };
@@ -308,7 +320,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k]);
         }
     };
-    // The entries' sources stream through the tile 64 at a time, as one
+    // The entries' sources stream through the tile TS at a time, as one
     // sequence over all entries of the item; the next tile's loads are
     // issued before the current tile is evaluated.
     int e = e0, j0 = 0;
@@ -318,18 +330,18 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     bool rel = false;
     if (e < e1) {
         rel = rel_entry(m);
-        next = fetch(m, lane, rel);
+        next = fetch(m, tid, rel);
     }
     while (e < e1) {
-        tile[lane] = next;
+        tile[tid] = next;
         __syncthreads();
-        const int nt = min(kLanes, m.z - j0);
+        const int nt = min(TS, m.z - j0);
         // With one tree for targets and sources, coincident points share a
         // leaf, so among point sources only the target leaf's own entry can
         // hold a zero-distance pair.  Proxy sources keep the test, except
         // those of separated boxes.
         const bool check = (!same_tree || m.x == T || (m.w & 1)) && !rel;
-        int en = e, jn = j0 + kLanes;
+        int en = e, jn = j0 + TS;
         int4 mn = m;
         bool reln = rel;
         if (jn >= m.z) {
@@ -338,70 +350,55 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
             while (en < e1 && (mn = ev.meta[en]).z == 0) en++;
             if (en < e1) reln = rel_entry(mn);
         }
-        if (en < e1) next = fetch(mn, jn + lane, reln);
-        float a32[TPT][C];
-#pragma unroll
-        for (int k = 0; k < TPT; k++)
-#pragma unroll
-            for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
-        auto run_tile = [&](auto relc) {
-            if (nt == kLanes && EXCL && !check) {
-                // no zero-distance pair is possible here: skip the exclusion test
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{}, relc);
-                }
-            } else if (nt == kLanes) {
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, relc);
-                }
-            } else {
-                for (int jj = 0; jj < nt; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, relc);
-                }
-            }
-        };
-        if constexpr (MIX) {
-            if (rel)
-                run_tile(std::true_type{});
-            else
-                run_tile(std::false_type{});
-        } else {
-            if (nt == kLanes && EXCL && !check) {
-                // no zero-distance pair is possible here: skip the exclusion test
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::false_type{}, std::false_type{});
-                }
-            } else if (nt == kLanes) {
-#pragma unroll 8
-                for (int jj = 0; jj < kLanes; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, std::false_type{});
-                }
-            } else {
-                for (int jj = 0; jj < nt; jj++) {
-                    const Tile s = tile[jj];
-#pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], std::bool_constant<EXCL>{}, std::false_type{});
-                }
-            }
-        }
-        if constexpr (F32) {
+        if (en < e1) next = fetch(mn, jn + tid, reln);
+        // The tile in chunks of 64 sources: each chunk's FP32 partial sums
+        // go into the FP64 accumulators, so the FP32 blocking stays at 64
+        // terms whatever the tile size.
+        auto run_chunk = [&](int c0, int cn, auto excl, auto relc) {
+            float a32[TPT][C];
 #pragma unroll
             for (int k = 0; k < TPT; k++)
 #pragma unroll
-                for (int c = 0; c < C; c++) acc[k][c] += (double)a32[k][c];
+                for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
+            if (cn == kLanes) {
+#pragma unroll 8
+                for (int jj = 0; jj < kLanes; jj++) {
+                    const Tile s = tile[c0 + jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], excl, relc);
+                }
+            } else {
+                for (int jj = 0; jj < cn; jj++) {
+                    const Tile s = tile[c0 + jj];
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], excl, relc);
+                }
+            }
+            if constexpr (F32) {
+#pragma unroll
+                for (int k = 0; k < TPT; k++)
+#pragma unroll
+                    for (int c = 0; c < C; c++) acc[k][c] += (double)a32[k][c];
+            }
+        };
+        auto run_tile = [&](auto relc) {
+            for (int c0 = 0; c0 < nt; c0 += kLanes) {
+                const int cn = min(kLanes, nt - c0);
+                if (EXCL && !check)  // no zero-distance pair is possible here: no exclusion test
+                    run_chunk(c0, cn, std::false_type{}, relc);
+                else
+                    run_chunk(c0, cn, std::bool_constant<EXCL>{}, relc);
+            }
+        };
+        if (active) {  // a wavefront without targets only helps fill the tiles
+            if constexpr (MIX) {
+                if (rel)
+                    run_tile(std::true_type{});
+                else
+                    run_tile(std::false_type{});
+            } else {
+                run_tile(std::false_type{});
+            }
         }
         __syncthreads();
         e = en;
@@ -411,7 +408,7 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     }
 #pragma unroll
     for (int k = 0; k < TPT; k++) {
-        const int ti = first + lane + kLanes * k;
+        const int ti = wfirst + lane + kLanes * k;
         if (ti < ntgt) {
             const size_t o = FAR ? (size_t)T * P + ti : (size_t)(tv.perm ? tv.perm[ts0 + ti] : ts0 + ti);
 #pragma unroll
@@ -514,7 +511,7 @@ struct InteractPlan {
     DBuf<int4> meta;
     struct Group {
         bool present = false;
-        int tpt = 1, nsplit = 1;
+        int tpt = 1, waves = 1, nsplit = 1;
         bool mutual = false;
         Tasks tk;
         Mutual mu;
@@ -530,7 +527,7 @@ struct InteractLaunch {
     EntryView ev;
     TgtView tv;
     SrcView sv;
-    int n_tasks, n1, tpt, far, excl, f32, atomic;
+    int n_tasks, n1, tpt, waves, far, excl, f32, atomic;
     int same_tree;  // targets and sources are one tree (the far group never compares panels)
     NodeTables tabs;
     KParams kp;
@@ -545,33 +542,39 @@ void launch_interact(const InteractLaunch &L, hipStream_t s);
 
 namespace btk {
 
-#define BTK_IK(F32_, TPT_, FAR_, EXCL_)                                                                          \
-    k_interact<DIM, KERNEL, F32_, TPT_, FAR_, EXCL_><<<L.n_tasks, kLanes, 0, s>>>(L.tk, L.ev, L.tv, L.sv, L.n1,  \
-                                                                                 L.tabs, L.kp, L.out,            \
-                                                                                 L.comp_stride, L.atomic, L.same_tree)
-#define BTK_IK_FE(F32_, TPT_)                                  \
-    do {                                                       \
-        if (L.far) {                                           \
-            if (L.excl) BTK_IK(F32_, TPT_, true, true);        \
-            else BTK_IK(F32_, TPT_, true, false);              \
-        } else {                                               \
-            if (L.excl) BTK_IK(F32_, TPT_, false, true);       \
-            else BTK_IK(F32_, TPT_, false, false);             \
-        }                                                      \
+#define BTK_IK(F32_, TPT_, FAR_, EXCL_, W_)                                                                 \
+    k_interact<DIM, KERNEL, F32_, TPT_, FAR_, EXCL_, W_><<<L.n_tasks, kLanes * W_, 0, s>>>(                  \
+        L.tk, L.ev, L.tv, L.sv, L.n1, L.tabs, L.kp, L.out, L.comp_stride, L.atomic, L.same_tree)
+#define BTK_IK_FE(F32_, TPT_, W_)                                  \
+    do {                                                           \
+        if (L.far) {                                               \
+            if (L.excl) BTK_IK(F32_, TPT_, true, true, W_);        \
+            else BTK_IK(F32_, TPT_, true, false, W_);              \
+        } else {                                                   \
+            if (L.excl) BTK_IK(F32_, TPT_, false, true, W_);       \
+            else BTK_IK(F32_, TPT_, false, false, W_);             \
+        }                                                          \
+    } while (0)
+#define BTK_IK_W(F32_, TPT_)                            \
+    do {                                                \
+        if (L.waves == 4) BTK_IK_FE(F32_, TPT_, 4);     \
+        else if (L.waves == 2) BTK_IK_FE(F32_, TPT_, 2); \
+        else BTK_IK_FE(F32_, TPT_, 1);                  \
     } while (0)
 
 template <int DIM, int KERNEL>
 void launch_interact(const InteractLaunch &L, hipStream_t s) {
     if (L.n_tasks <= 0) return;
     if (!L.f32)
-        BTK_IK_FE(false, 1);
+        BTK_IK_W(false, 1);
     else if (L.tpt == 2)
-        BTK_IK_FE(true, 2);
+        BTK_IK_W(true, 2);
     else
-        BTK_IK_FE(true, 1);
+        BTK_IK_W(true, 1);
     BTK_LAUNCH_CHECK();
 }
 
+#undef BTK_IK_W
 #undef BTK_IK_FE
 #undef BTK_IK
 

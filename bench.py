@@ -118,7 +118,7 @@ def parse():
     ap.add_argument("--mac-target", type=float, default=0.85e-6,
                     help="learned MAC: rel-L2 the calibration asks of every held-out set (margin below 1e-6)")
     ap.add_argument("--mac-holdout", type=int, default=2, help="learned MAC: held-out sets for the calibration")
-    ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL",
+    ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES]",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline up to 1M points: every k-th block of 64 PP/PC pairs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -129,6 +129,13 @@ def parse():
 # per-pair error budgets of the learned MAC the calibration chooses from:
 # 2e-3 down to 2e-6 in steps of 10^(1/12) (about 21 %)
 MAC_BUDGETS = tuple(2e-3 * 10 ** (-k / 12) for k in range(37))
+
+
+def shape_args(text):
+    """--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES] as set_tuning keywords."""
+    keys = ("near_items", "far_items", "near_tpt", "far_tpt", "mutual", "near_waves", "far_waves")
+    vals = [int(x) for x in text.split(",")]
+    return dict(zip(keys, vals))
 
 
 def make_points(gen, n, seed):
@@ -620,7 +627,7 @@ def main():
             btk.mac_set_budget(budget)
         btk.set_tuning()  # defaults
         if args.shapes:
-            btk.set_tuning(*[int(x) for x in args.shapes.split(",")])
+            btk.set_tuning(**shape_args(args.shapes))
         elif not args.no_autotune:  # untimed, like the warmup
             log(f"autotune {v}")
             btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
@@ -632,8 +639,8 @@ def main():
         if v == "learned":
             res[v].update(mac_budget=budget, mac_budget_choice=choice, mac_training=train)
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
-                                   "far_tpt": t.far_tpt, "mutual": t.mutual,
-                                   "autotuned": not (args.no_autotune or args.shapes)}
+                                   "far_tpt": t.far_tpt, "mutual": t.mutual, "near_waves": t.near_waves,
+                                   "far_waves": t.far_waves, "autotuned": not (args.no_autotune or args.shapes)}
         log(f"{v}: {ms:.2f} ms/solve, rel-L2 {res[v].get('rel_l2')}")
     # the fastest variant that meets the metric's error
     meets = [v for v in res if ds is None or res[v]["rel_l2"] <= 1e-6]

@@ -131,15 +131,20 @@ typedef struct btk_tuning {
     int cc_group;   /* mixed precision, 3D, potential (+gradient): CC pairs with the same two levels and
                        box-centre offset share their kernel matrix; a set of at least cc_group such pairs
                        runs as one FP32 MFMA product.  0 = default (8), -1 = off */
-    int reserved[2];
+    int near_waves; /* wavefronts per workgroup of the PP+PC kernel, 1, 2 or 4 (0 = default: 4 in 3D, 1 in
+                       2D): they take
+                       consecutive blocks of a panel's targets and share one LDS tile of 64 * near_waves
+                       sources, so each source is read from memory once per workgroup */
+    int far_waves;  /* the same for CP+CC (0 = default, 1) */
 } btk_tuning;
 int btk_set_tuning(const btk_tuning *t);
 int btk_get_tuning(btk_tuning *t);
 
 /* Autotuner (no reference counterpart): builds the trees and list of one
  * problem (the arguments of btk_solve), times the PP+PC and CP+CC launches
- * under each candidate shape (work-item targets 8192..131072; in mixed
- * precision also 1 or 2 targets per lane and mutual on or off), installs
+ * under each candidate shape (work-item targets 8192..131072, 1, 2 or 4
+ * wavefronts per workgroup; in mixed precision also 1 or 2 targets per lane
+ * and mutual on or off), installs
  * the fastest with btk_set_tuning and returns it in *chosen (may be null).
  * Writes no caller memory besides *chosen. */
 int btk_autotune(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,

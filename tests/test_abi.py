@@ -51,8 +51,13 @@ def test_tuning_round_trip_and_rejects_bad_values():
             btk.set_tuning(near_tpt=3)
         with pytest.raises(btk.BtkError, match="work item"):
             btk.set_tuning(far_items=-1)
+        btk.set_tuning(near_waves=4, far_waves=2)
+        t = btk.get_tuning()
+        assert (t.near_waves, t.far_waves) == (4, 2)
+        with pytest.raises(btk.BtkError, match="wavefronts per workgroup"):
+            btk.set_tuning(near_waves=3)
     finally:
-        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual)
+        btk.restore_tuning(old)
 
 
 def test_components():

@@ -124,7 +124,7 @@ def test_learned_mixed_precision_vs_oracle_degree_by_degree(cc_group):
         st = btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, mac=1, precision=btk.PREC_MIXED),
                        d[0], d[1], d[2], d[3], out)
     finally:
-        btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual, old.cc_group)
+        btk.restore_tuning(old)
     print(f"  CC pairs {int(st.count[3])}, as shared-matrix products {int(st.cc_shared)}", flush=True)
     if cc_group == -1:
         assert st.cc_shared == 0

@@ -32,7 +32,7 @@ MIXED_TOL = 1e-6
 def _restore_tuning():
     old = btk.get_tuning()
     yield
-    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual, old.cc_group)
+    btk.restore_tuning(old)
 
 
 def points(kind, n, seed):
@@ -80,6 +80,8 @@ SHAPES = [
     ("mutual_pairs", {"near_tpt": 1, "mutual": 1}),
     ("cc_pairwise", {"cc_group": -1}),
     ("cc_shared_from_2", {"cc_group": 2}),
+    ("four_waves_per_tile", {"near_waves": 4, "far_waves": 4}),
+    ("two_waves_two_targets_per_lane", {"near_waves": 2, "far_waves": 2, "near_tpt": 2, "far_tpt": 2}),
 ]
 
 
@@ -105,6 +107,10 @@ def test_solves_vs_oracle(kind, n, dim, deg, leaf, theta, eps, kernel, excl):
     assert [int(st.count[k]) for k in range(4)] == cnt
     e = rel(got, ref)
     print(f"  pairs {cnt}; FP64 vs oracle {e:.2e}", flush=True)
+    assert e < FP64_TOL
+    got, st = run(btk.PREC_FP64, near_waves=4, far_waves=2)
+    e = rel(got, ref)
+    print(f"  FP64, 4 and 2 wavefronts per tile, vs oracle {e:.2e}", flush=True)
     assert e < FP64_TOL
     for name, shape in SHAPES:
         if dim == 2 and "cc_group" in shape:

@@ -18,7 +18,7 @@ ITEMS = {8192, 16384, 32768, 65536, 131072}
 def _restore_tuning():
     old = btk.get_tuning()
     yield
-    btk.set_tuning(old.near_items, old.far_items, old.near_tpt, old.far_tpt, old.mutual, old.cc_group)
+    btk.restore_tuning(old)
 
 
 def _problem(dim, n, seed):
@@ -37,9 +37,10 @@ def test_autotune_installs_a_candidate(dim, prec):
     o = btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=prec)
     t = btk.autotune(cfg, o, d[0], d[1], d[2], d[3])
     g = btk.get_tuning()
-    assert (g.near_items, g.far_items, g.near_tpt, g.far_tpt, g.mutual) == \
-        (t.near_items, t.far_items, t.near_tpt, t.far_tpt, t.mutual)
+    assert (g.near_items, g.far_items, g.near_tpt, g.far_tpt, g.mutual, g.near_waves, g.far_waves) == \
+        (t.near_items, t.far_items, t.near_tpt, t.far_tpt, t.mutual, t.near_waves, t.far_waves)
     assert t.near_items in ITEMS and t.far_items in ITEMS
+    assert t.near_waves in (1, 2, 4) and t.far_waves in (1, 2, 4)
     if prec == btk.PREC_MIXED:
         assert t.near_tpt in (1, 2) and t.far_tpt in (1, 2) and t.mutual in (0, 1)
     else:
@@ -54,7 +55,8 @@ def test_results_independent_of_launch_shapes(dim):
     C = btk.components(dim, btk.KERNEL_POT_GRAD)
     outs = {}
     for prec in (btk.PREC_FP64, btk.PREC_MIXED):
-        for shape in [(0, 0, 0, 0, 0), (8192, 8192, 1, 1, 0), (131072, 131072, 2, 2, 1), (16384, 65536, 1, 2, 1)]:
+        for shape in [(0, 0, 0, 0, 0), (8192, 8192, 1, 1, 0), (131072, 131072, 2, 2, 1), (16384, 65536, 1, 2, 1),
+                      (8192, 16384, 1, 1, 0, 0, 4, 4), (32768, 8192, 2, 1, 1, 0, 2, 4)]:
             btk.set_tuning(*shape)
             out = torch.zeros(C * 40000, dtype=torch.float64, device="cuda")
             btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, precision=prec), d[0], d[1],
