@@ -697,8 +697,10 @@ def main():
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size) of the
     # same kernel shape (mutual pairs or not, targets per lane)
     sh = main_r.get("launch_shapes", {}) if main_r else {}
+    waves = sh.get("near_waves", 0) or (1 if dim == 2 else 4)
     tfile = os.path.join(HERE, "profiles", f"traffic_k_interact_near_{args.workload}_{head}"
-                                           f"_m{sh.get('mutual', 0)}_t{sh.get('near_tpt', 0) or (2 if dim == 2 else 1)}.json")
+                                           f"_m{sh.get('mutual', 0)}_t{sh.get('near_tpt', 0) or (2 if dim == 2 else 1)}"
+                                           f"_w{waves}.json")
     if main_r and os.path.exists(tfile):
         t = json.load(open(tfile))
         line["roofline"] = dict(line["roofline"], traffic=t["traffic_bytes_per_launch"],
