@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libbtk.so")
 
 KERNEL_POT, KERNEL_REF_VEL3D, KERNEL_POT_GRAD = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+SCHEDULE_GROUPS, SCHEDULE_QUEUE = 0, 1
 
 EXPORTED = [
     "btk_components", "btk_last_error", "btk_release_cached", "btk_set_tuning", "btk_get_tuning",
@@ -36,7 +37,7 @@ EXPORTED = [
     "btk_upward_pass_3d_noalloc",
     "btk_fmm_interactions_2d", "btk_fmm_interactions_3d", "btk_downward_pass_2d", "btk_downward_pass_3d",
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
-    "btk_plan_create", "btk_plan_execute", "btk_plan_destroy",
+    "btk_plan_create", "btk_plan_create_batch", "btk_plan_execute", "btk_plan_destroy",
     "btk_mac_sample", "btk_mac_train", "btk_mac_set_weights", "btk_mac_get_weights", "btk_mac_set_budget",
     "btk_mac_forward", "btk_mac_gradients",
 ]
@@ -63,7 +64,8 @@ class TreeInfo(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
-                ("deepest_only", C.c_int), ("precision", C.c_int), ("mac", C.c_int), ("reserved", C.c_int * 2)]
+                ("deepest_only", C.c_int), ("precision", C.c_int), ("mac", C.c_int), ("schedule", C.c_int),
+                ("reserved", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -161,6 +163,7 @@ def lib():
     L.btk_mac_forward.argtypes = [fp, C.c_int, fp, vp]
     L.btk_mac_gradients.argtypes = [fp, fp, C.c_int, fp, C.POINTER(C.c_float), vp]
     L.btk_plan_create.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
+    L.btk_plan_create_batch.argtypes = [cfgp, optp, C.c_int, ip, dp, dp, dp, vp, C.POINTER(vp)]
     L.btk_plan_execute.argtypes = [vp, dp, dp, vp]
     L.btk_plan_destroy.argtypes = [vp]
     L.btk_plan_destroy.restype = None
@@ -220,10 +223,11 @@ def make_config(dim, degree, thresh, theta=0.7, eps=-1.0) -> RunConfig:
     return c
 
 
-def options(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, precision=PREC_FP64, mac=0) -> Options:
+def options(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, precision=PREC_FP64, mac=0,
+            schedule=0) -> Options:
     o = Options()
     o.kernel, o.convention, o.exclude_self = kernel, convention, exclude_self
-    o.deepest_only, o.precision, o.mac = deepest_only, precision, mac
+    o.deepest_only, o.precision, o.mac, o.schedule = deepest_only, precision, mac, schedule
     return o
 
 
@@ -470,6 +474,19 @@ class Plan:
         _check(lib().btk_plan_create(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
                                      int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), int(coincident), _stream(stream),
                                      C.byref(self.h)))
+
+    @classmethod
+    def batch(cls, cfg, opts, offsets, x, y, z=None, stream=None):
+        """A plan for a fixed forest (btk_plan_create_batch); problem p owns
+        points offsets[p] .. offsets[p+1]."""
+        self = cls.__new__(cls)
+        off = np.ascontiguousarray(np.asarray(offsets), np.int32)
+        self._keep = (x, y, z, off)
+        self.h = C.c_void_p()
+        _check(lib().btk_plan_create_batch(C.byref(cfg), C.byref(opts), len(off) - 1,
+                                           off.ctypes.data_as(C.POINTER(C.c_int)), _ptr(x), _ptr(y), _ptr(z),
+                                           _stream(stream), C.byref(self.h)))
+        return self
 
     def execute(self, q, out, stream=None):
         _check(lib().btk_plan_execute(self.h, _ptr(q), _ptr(out), _stream(stream)))

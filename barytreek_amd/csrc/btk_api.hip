@@ -802,6 +802,37 @@ int btk_plan_create(const btk_run_config *cfg_in, const btk_options *opts, int m
     });
 }
 
+int btk_plan_create_batch(const btk_run_config *cfg_in, const btk_options *opts, int nprob, const int *offsets,
+                          const double *d_x, const double *d_y, const double *d_z, void *stream, btk_plan **plan) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in && offsets && plan, "null argument");
+        BTK_REQUIRE(nprob >= 1, "nprob must be positive");
+        auto p = std::make_unique<btk_plan>();
+        p->cfg = *cfg_in;
+        const int dim = p->cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        p->o = opts_or_default(opts);
+        const std::vector<int> off(offsets, offsets + nprob + 1);
+        BTK_REQUIRE(off[0] == 0, "offsets[0] must be 0");
+        for (int k = 0; k < nprob; k++) BTK_REQUIRE(off[k + 1] > off[k], "every problem needs at least one point");
+        BTK_REQUIRE(!(p->o.deepest_only && nprob > 1), "deepest_only is defined for a single problem only");
+        hipStream_t s = as_stream(stream);
+        p->ts = btk::build_forest(dim, p->cfg.fmm_cluster_thresh, d_x, d_y, d_z, off, s);
+        p->tt = p->ts.get();
+        p->l = btk::build_list(&p->cfg, *p->tt, *p->ts, s, p->o.mac != 0);
+        p->ip = btk::plan_interactions(p->cfg, p->o, *p->tt, *p->ts, *p->l, s, true);
+        p->m = p->n = p->ts->n;
+        p->C = btk::components(dim, p->o.kernel);
+        p->q.resize(p->n ? p->n : 1);
+        p->W.resize(btk::proxy_offset(*p->l, p->ts->pc, 1, p->l->ndeg));
+        p->Psi.resize(btk::proxy_offset(*p->l, p->tt->pc, p->C, p->l->ndeg));
+        p->outs.resize((size_t)p->C * (p->m ? p->m : 1));
+        BTK_HIP(hipStreamCreateWithFlags(&p->cs, hipStreamNonBlocking));
+        BTK_HIP(hipStreamSynchronize(s));
+        *plan = p.release();
+    });
+}
+
 int btk_plan_execute(btk_plan *p, const double *d_q, double *d_out, void *stream) {
     return guarded([&] {
         BTK_REQUIRE(p, "null plan");

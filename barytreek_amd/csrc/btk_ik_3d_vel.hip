@@ -3,4 +3,5 @@
 
 namespace btk {
 template void launch_interact<3, BTK_KERNEL_REF_VEL3D>(const InteractLaunch &, hipStream_t);
+template void launch_queue<3, BTK_KERNEL_REF_VEL3D>(const QueueLaunch &, hipStream_t);
 }  // namespace btk

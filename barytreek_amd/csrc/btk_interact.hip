@@ -154,16 +154,75 @@ __global__ void __launch_bounds__(kLanes)
     }
 }
 
+// Estimated cost of every work item of a queue (targets x sources of its
+// entry slice), as a key that sorts the biggest first, and its flattened
+// number as the value.
+__global__ void k_queue_cost(int n, int ngroups, const QueueGroup *__restrict__ groups, int dim,
+                             const int4 *__restrict__ meta, const int *__restrict__ tcount,
+                             const int *__restrict__ tpt, unsigned long long *__restrict__ key,
+                             int *__restrict__ val) {
+    const int f = blockIdx.x * TB + threadIdx.x;
+    if (f >= n) return;
+    int g = 0;
+    while (g + 1 < ngroups && groups[g + 1].first <= f) g++;
+    const QueueGroup &G = groups[g];
+    const int k = f - G.first;
+    const int P = dim == 3 ? G.n1 * G.n1 * G.n1 : G.n1 * G.n1;
+    const int nt = G.far ? P : tcount[G.tk.panel[k]];
+    const int per = kLanes * tpt[G.far];
+    const long long targets = min(per, nt - G.tk.first[k]);
+    long long sources = 0;
+    for (int e = G.tk.e0[k]; e < G.tk.e1[k]; e++) sources += meta[e].z;
+    key[f] = ~(unsigned long long)(targets * sources);
+    val[f] = f;
+}
+
+__global__ void k_queue_order(int n, int ngroups, const QueueGroup *__restrict__ groups, const int *__restrict__ val,
+                              int2 *__restrict__ order) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const int f = val[i];
+    int g = 0;
+    while (g + 1 < ngroups && groups[g + 1].first <= f) g++;
+    order[i] = make_int2(g, f - groups[g].first);
+}
+
 template <int DIM, int KERNEL>
 void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, double *out, double *Psi,
                 hipStream_t s, KernelEvents *kev) {
     const Tree &tt = *p.tt, &ts = *p.ts;
     if (kev) kev->cc_shared = p.cc_shared;
+    const bool queued = p.queue.n > 0;
+    if (queued) {
+        // every PP/PC/CP/CC work item in one persistent launch ("near" time);
+        // the shared-matrix CC products and mutual pairs follow ("far" time)
+        QueueLaunch Q;
+        Q.q = &p.queue;
+        Q.ev = EntryView{p.meta.p};
+        Q.tv = TgtView{tt.start.p, tt.count.p, p.sorted_out ? nullptr : tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
+                       tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
+        Q.sv = SrcView{ts.start.p, ts.count.p, ts.sx.p, ts.sy.p, ts.sz.p, q_sorted, ts.lo[0].p, ts.lo[1].p,
+                       ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, W};
+        Q.kp = p.kp;
+        Q.same_tree = &tt == &ts ? 1 : 0;
+        Q.excl = p.o.exclude_self ? 1 : 0;
+        Q.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
+        Q.W = W;
+        Q.out = out;
+        Q.Psi = Psi;
+        if (kev) BTK_HIP(hipEventRecord(kev->ev[0], s));
+        launch_queue<DIM, KERNEL>(Q, s);
+        if (kev) {
+            BTK_HIP(hipEventRecord(kev->ev[1], s));
+            kev->near_done = true;
+            BTK_HIP(hipEventRecord(kev->ev[2], s));
+        }
+    }
     for (int far = 0; far < 2; far++) {
         bool any = false;
         for (int di = 0; di < p.ndeg; di++) any |= p.g[far][di].present;
         if (!any) continue;
-        if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
+        if (kev && !queued) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
         for (int di = 0; di < p.ndeg; di++) {
             const InteractPlan::Group &g = p.g[far][di];
             if (!g.present) continue;
@@ -194,12 +253,16 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, p.sorted_out, s);
             if (g.cc.n_items)
                 launch_ccgemm(g.cc, tt, ts, W + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride, s);
-            if (g.tk.n) launch_interact<DIM, KERNEL>(L, s);
+            if (g.tk.n && !queued) launch_interact<DIM, KERNEL>(L, s);
         }
-        if (kev) {
+        if (kev && !queued) {
             BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
             (far ? kev->far_done : kev->near_done) = true;
         }
+    }
+    if (kev && queued) {
+        BTK_HIP(hipEventRecord(kev->ev[3], s));
+        kev->far_done = true;
     }
 }
 
@@ -293,10 +356,67 @@ btk_tuning &tuning() {
     return t;
 }
 
+// Resident workgroups of a kernel on the whole GPU (a persistent grid).
+int queue_grid(const void *kernel) {
+    int dev = 0, cus = 0, nb = 0;
+    BTK_HIP(hipGetDevice(&dev));
+    BTK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BTK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kLanes, 0));
+    return std::max(1, nb) * std::max(1, cus);
+}
+
 int choose_split(long long base_items, long long target_items) {
     if (base_items <= 0) return 1;
     long long k = (target_items + base_items - 1) / base_items;
     return (int)std::max(1ll, std::min<long long>(kMaxSplit, k));
+}
+
+// The queue of BTK_SCHEDULE_QUEUE: the work items of every group, biggest
+// first.  Near groups of several degrees write the same outputs, so their
+// items then add atomically.
+void build_queue(InteractPlan &p, const List &l, const Tree &tt, hipStream_t s) {
+    BTK_REQUIRE(p.o.schedule == BTK_SCHEDULE_QUEUE, "not a queued plan");
+    std::vector<QueueGroup> hg;
+    int total = 0, near_groups = 0;
+    for (int di = 0; di < p.ndeg; di++) near_groups += p.g[0][di].tk.n > 0;
+    for (int far = 0; far < 2; far++)
+        for (int di = 0; di < p.ndeg; di++) {
+            const InteractPlan::Group &g = p.g[far][di];
+            if (g.tk.n == 0) continue;
+            const int n1 = p.deg[di] + 1;
+            QueueGroup q{};
+            q.tk = TaskView{g.tk.panel.p, g.tk.first.p, g.tk.e0.p, g.tk.e1.p};
+            q.woff = p.woff[di];
+            q.ooff = far ? p.poff[di] : 0;
+            q.comp_stride = far ? (size_t)tt.pc * ipow(n1, tt.dim) : (size_t)tt.n;
+            q.n1 = n1;
+            q.far = far;
+            q.atomic = (g.nsplit > 1 || (!far && near_groups > 1)) ? 1 : 0;
+            q.first = total;
+            q.tabs = p.tabs[di];
+            (far ? p.queue.ftpt : p.queue.ntpt) = g.tpt;
+            total += g.tk.n;
+            hg.push_back(q);
+        }
+    p.queue.n = total;
+    if (total == 0) return;
+    const int ng = (int)hg.size();
+    p.queue.groups.resize(ng);
+    p.queue.order.resize(total);
+    p.queue.ticket.resize(1);
+    BTK_HIP(hipMemcpyAsync(p.queue.groups.p, hg.data(), sizeof(QueueGroup) * ng, hipMemcpyHostToDevice, s));
+    const int htpt[2] = {p.queue.ntpt, p.queue.ftpt};
+    DBuf<int> tpt(2);
+    BTK_HIP(hipMemcpyAsync(tpt.p, htpt, sizeof htpt, hipMemcpyHostToDevice, s));
+    DBuf<unsigned long long> key(total), key2(total);
+    DBuf<int> val(total), val2(total);
+    k_queue_cost<<<nblk(total), TB, 0, s>>>(total, ng, p.queue.groups.p, tt.dim, p.meta.p, tt.count.p, tpt.p, key.p,
+                                            val.p);
+    BTK_LAUNCH_CHECK();
+    sort_pairs_u64(key.p, key2.p, val.p, val2.p, total, 64, s);
+    k_queue_order<<<nblk(total), TB, 0, s>>>(total, ng, p.queue.groups.p, val2.p, p.queue.order.p);
+    BTK_LAUNCH_CHECK();
+    BTK_HIP(hipStreamSynchronize(s));  // hg and htpt are host memory of this call
 }
 
 std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const btk_options &o, const Tree &tt,
@@ -306,6 +426,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
     const int n1 = cfg.interp_degree + 1;
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
     BTK_REQUIRE(o.precision == BTK_PREC_FP64 || o.precision == BTK_PREC_MIXED, "unknown precision");
+    BTK_REQUIRE(o.schedule == BTK_SCHEDULE_GROUPS || o.schedule == BTK_SCHEDULE_QUEUE, "unknown schedule");
     BTK_REQUIRE(!(dim == 2 && o.kernel == BTK_KERNEL_REF_VEL3D), "the reference vel kernel is 3D only");
     BTK_REQUIRE(l.ndeg >= 1 && l.ndeg <= 4 && l.deg[l.ndeg - 1] == cfg.interp_degree,
                 "the list was built for another interp_degree");
@@ -362,6 +483,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             // cluster: one wavefront each
             const int waves_set = far ? tu.far_waves : tu.near_waves;
             g.waves = waves_set ? waves_set : ((far || dim == 2) ? 1 : 4);
+            if (o.schedule == BTK_SCHEDULE_QUEUE) g.waves = 1;  // the persistent kernel's workgroup
             const int per = kLanes * g.tpt * g.waves;
             // mixed precision, one tree, when asked for: mutual PP pairs once
             // for both directions (PP entries are in the degree-0 near group)
@@ -389,6 +511,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             g.present = g.tk.n > 0 || g.mutual || g.cc.n_items > 0;
         }
     }
+    if (o.schedule == BTK_SCHEDULE_QUEUE) build_queue(*p, l, tt, s);
     return p;
 }
 

@@ -29,6 +29,7 @@
 // into FP64 per-target accumulators.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "btk_device.hpp"
@@ -218,23 +219,25 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
     return (r / kRun) * (kXcds * kRun) + x * kRun + (r % kRun);
 }
 
-// The mixed-precision kernels ask for at least 6 wavefronts per SIMD (at most
-// 80 VGPRs); left to itself the compiler spends up to 113 on unrolling and
-// runs 4.  The FP64 kernels keep the compiler's choice.
+// The LDS tile type of an instance.
+template <int DIM, bool F32, bool FAR>
+using TileOf = std::conditional_t<F32 && FAR, TileRel, std::conditional_t<F32 && !FAR, TileMix<DIM>, TileSrc<F32>>>;
+
+// One work item, evaluated by the whole workgroup (64 * WAVES threads)
+// through the LDS tile `tile` of 64 * WAVES sources.
 template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES>
-__global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
-    k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
-               double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
+__device__ __forceinline__ void interact_item(int item, const TaskView &tk, const EntryView &ev, const TgtView &tv,
+                                              const SrcView &sv, int n1, const NodeTables &tabs, const KParams &kp,
+                                              double *__restrict__ out, size_t comp_stride, int atomic,
+                                              int same_tree, TileOf<DIM, F32, FAR> *tile) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
     constexpr bool MIX = F32 && !FAR;                // FP64 or relative FP32 coordinates (TileMix)
     constexpr int TS = kLanes * WAVES;               // sources per tile
-    using Tile = std::conditional_t<REL, TileRel, std::conditional_t<MIX, TileMix<DIM>, TileSrc<F32>>>;
-    __shared__ Tile tile[TS];
+    using Tile = TileOf<DIM, F32, FAR>;
     const int tid = threadIdx.x;
     const int lane = tid % kLanes;
     const int wave = tid / kLanes;
-    const int item = xcd_contiguous(blockIdx.x, gridDim.x);
     const int T = tk.panel[item];
     const int first = tk.first[item];
     const int e0 = tk.e0[item], e1 = tk.e1[item];
@@ -426,6 +429,59 @@ __global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
     }
 }
 
+// The mixed-precision kernels ask for at least 6 wavefronts per SIMD (at most
+// 80 VGPRs); left to itself the compiler spends up to 113 on unrolling and
+// runs 4.  The FP64 kernels keep the compiler's choice.
+template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES>
+__global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
+    k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
+               double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
+    __shared__ TileOf<DIM, F32, FAR> tile[kLanes * WAVES];
+    interact_item<DIM, KERNEL, F32, TPT, FAR, EXCL, WAVES>(xcd_contiguous(blockIdx.x, gridDim.x), tk, ev, tv, sv, n1,
+                                                           tabs, kp, out, comp_stride, atomic, same_tree, tile);
+}
+
+// One group (near or far, one degree) of a work queue.  Its source proxy
+// weights are W + woff, its outputs out + ooff (near) or Psi + ooff (far).
+struct QueueGroup {
+    TaskView tk;
+    size_t woff, ooff, comp_stride;
+    int n1, far, atomic, first;  // first: the group's first item in the flattened numbering
+    NodeTables tabs;
+};
+
+// Persistent interactions: a grid sized to the GPU's resident workgroups
+// drains one queue of work items of every near and far group (biggest
+// first), taking the next item by atomic ticket.  *ticket must be zero at
+// launch; every workgroup leaves once the ticket passes n.
+template <int DIM, int KERNEL, bool F32, int NTPT, int FTPT, bool EXCL>
+__global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
+    k_interact_queue(const QueueGroup *__restrict__ groups, const int2 *__restrict__ queue, int n,
+                     int *__restrict__ ticket, EntryView ev, TgtView tv, SrcView sv, KParams kp, int same_tree,
+                     const double *__restrict__ W, double *__restrict__ out, double *__restrict__ Psi) {
+    __shared__ TileOf<DIM, F32, false> tn[kLanes];
+    __shared__ TileOf<DIM, F32, true> tf[kLanes];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int q = s_item;
+        __syncthreads();  // every lane has read s_item before the next ticket overwrites it
+        if (q >= n) break;
+        const int2 gi = queue[q];
+        const QueueGroup &g = groups[gi.x];
+        SrcView s2 = sv;
+        s2.W = W + g.woff;
+        if (g.far)
+            interact_item<DIM, KERNEL, F32, FTPT, true, EXCL, 1>(gi.y, g.tk, ev, tv, s2, g.n1, g.tabs, kp, Psi + g.ooff,
+                                                                 g.comp_stride, g.atomic, 0, tf);
+        else
+            interact_item<DIM, KERNEL, F32, NTPT, false, EXCL, 1>(gi.y, g.tk, ev, tv, s2, g.n1, g.tabs, kp,
+                                                                  out + g.ooff, g.comp_stride, g.atomic, same_tree,
+                                                                  tn);
+    }
+}
+
 // Work items of one entry group of the list (near: PP+PC, far: CP+CC, at one
 // degree): blocks of `per` targets of a panel times slices of its entries,
 // balanced by source count.
@@ -519,6 +575,14 @@ struct InteractPlan {
     } g[2][4];  // [far][degree index]
     long long cc_shared = 0;  // CC entries evaluated through shared kernel matrices
     bool sorted_out = false;  // near output in the target tree's sorted order
+    // btk_options.schedule = BTK_SCHEDULE_QUEUE: the work items of every
+    // group as one queue, biggest first, drained by one persistent launch
+    struct Queue {
+        int n = 0, ntpt = 1, ftpt = 1;
+        DBuf<QueueGroup> groups;
+        DBuf<int2> order;
+        DBuf<int> ticket;
+    } queue;
 };
 
 // Launch of one interaction group with everything fixed but the kernel.
@@ -537,6 +601,21 @@ struct InteractLaunch {
 
 template <int DIM, int KERNEL>
 void launch_interact(const InteractLaunch &L, hipStream_t s);
+
+// The persistent launch of a plan's queue (targets, sources and the kernel
+// parameters as for the group launches).
+struct QueueLaunch {
+    const InteractPlan::Queue *q;
+    EntryView ev;
+    TgtView tv;
+    SrcView sv;
+    KParams kp;
+    int same_tree, excl, f32;
+    const double *W;
+    double *out, *Psi;
+};
+template <int DIM, int KERNEL>
+void launch_queue(const QueueLaunch &L, hipStream_t s);
 
 }  // namespace btk
 
@@ -578,6 +657,51 @@ void launch_interact(const InteractLaunch &L, hipStream_t s) {
 #undef BTK_IK_FE
 #undef BTK_IK
 
+namespace {
+__global__ void k_ticket_reset(int *p) { *p = 0; }
+}  // namespace
+int queue_grid(const void *kernel);
+
+#define BTK_IQ(F32_, NT_, FT_, EXCL_)                                                                          \
+    do {                                                                                                      \
+        auto kf = k_interact_queue<DIM, KERNEL, F32_, NT_, FT_, EXCL_>;                                        \
+        const int g = std::min(L.q->n, queue_grid(reinterpret_cast<const void *>(kf)));                       \
+        kf<<<g, kLanes, 0, s>>>(L.q->groups.p, L.q->order.p, L.q->n, L.q->ticket.p, L.ev, L.tv, L.sv, L.kp,    \
+                                L.same_tree, L.W, L.out, L.Psi);                                              \
+    } while (0)
+#define BTK_IQ_E(F32_, NT_, FT_)                    \
+    do {                                            \
+        if (L.excl) BTK_IQ(F32_, NT_, FT_, true);   \
+        else BTK_IQ(F32_, NT_, FT_, false);         \
+    } while (0)
+
+template <int DIM, int KERNEL>
+void launch_queue(const QueueLaunch &L, hipStream_t s) {
+    if (L.q->n <= 0) return;
+    k_ticket_reset<<<1, 1, 0, s>>>(L.q->ticket.p);  // a kernel, not a memset: the launch may be captured in a graph
+    BTK_LAUNCH_CHECK();
+    const int nt = L.q->ntpt, ft = L.q->ftpt;
+    if (!L.f32)
+        BTK_IQ_E(false, 1, 1);
+    else if (nt == 1 && ft == 1)
+        BTK_IQ_E(true, 1, 1);
+    else if (nt == 1)
+        BTK_IQ_E(true, 1, 2);
+    else if (ft == 1)
+        BTK_IQ_E(true, 2, 1);
+    else
+        BTK_IQ_E(true, 2, 2);
+    BTK_LAUNCH_CHECK();
+}
+
+#undef BTK_IQ_E
+#undef BTK_IQ
+
+extern template void launch_queue<2, BTK_KERNEL_POT>(const QueueLaunch &, hipStream_t);
+extern template void launch_queue<2, BTK_KERNEL_POT_GRAD>(const QueueLaunch &, hipStream_t);
+extern template void launch_queue<3, BTK_KERNEL_POT>(const QueueLaunch &, hipStream_t);
+extern template void launch_queue<3, BTK_KERNEL_REF_VEL3D>(const QueueLaunch &, hipStream_t);
+extern template void launch_queue<3, BTK_KERNEL_POT_GRAD>(const QueueLaunch &, hipStream_t);
 extern template void launch_interact<2, BTK_KERNEL_POT>(const InteractLaunch &, hipStream_t);
 extern template void launch_interact<2, BTK_KERNEL_POT_GRAD>(const InteractLaunch &, hipStream_t);
 extern template void launch_interact<3, BTK_KERNEL_POT>(const InteractLaunch &, hipStream_t);

@@ -110,6 +110,8 @@ def parse():
     ap.add_argument("--mac-sets", type=int, default=10, help="learned MAC: Plummer sets sampled for training")
     ap.add_argument("--mac-pairs", type=int, default=100_000, help="learned MAC: cluster pairs sampled per set")
     ap.add_argument("--mac-epochs", type=int, default=20, help="learned MAC: training epochs")
+    ap.add_argument("--schedule", choices=["groups", "queue"], default="groups",
+                    help="interaction work items: one launch per group, or one persistent work-queue launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -454,8 +456,10 @@ def bench_batched(args, btk, torch, wl, dev):
 
 def bench_forest(args, btk, torch, wl, dev):
     """Many independent problems: all of them in one forest solve
-    (btk_solve_batch, FP64 and mixed) against the same problems solved one
-    by one with btk_solve (FP64)."""
+    (btk_solve_batch; mixed precision with the persistent work-queue launch,
+    and with one launch per group; FP64) against the same problems solved one
+    by one with btk_solve (FP64), plus the forest as a plan replayed as one
+    HIP graph per charge vector."""
     dim, nprob, lo, hi, a, degree, theta, leaf, desc = FOREST[args.workload]
     rng = np.random.default_rng(SEED + 11)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), nprob)).astype(np.int64)
@@ -467,8 +471,8 @@ def bench_forest(args, btk, torch, wl, dev):
     cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
     kernel = btk.KERNEL_POT
 
-    def forest(prec, steps, warmup):
-        o = btk.options(kernel=kernel, exclude_self=1, precision=prec)
+    def forest(prec, steps, warmup, schedule=0):
+        o = btk.options(kernel=kernel, exclude_self=1, precision=prec, schedule=schedule)
         out = torch.zeros(n, dtype=torch.float64, device=dev)
         for _ in range(warmup):
             out.zero_()
@@ -501,8 +505,27 @@ def bench_forest(args, btk, torch, wl, dev):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, out
 
-    t_m, out_m, st_m = forest(btk.PREC_MIXED, args.steps, args.warmup)
+    t_g, out_g, st_g = forest(btk.PREC_MIXED, args.steps, args.warmup)
+    t_m, out_m, st_m = forest(btk.PREC_MIXED, args.steps, args.warmup, schedule=btk.SCHEDULE_QUEUE)
     t_p, out_p, st_p = forest(btk.PREC_FP64, args.steps, args.warmup)
+
+    # the same forest as a plan: trees, list and work items once, then each
+    # charge vector is one HIP graph replay (persistent work-queue launch)
+    plan = btk.Plan.batch(cfg, btk.options(kernel=kernel, exclude_self=1, precision=btk.PREC_MIXED,
+                                           schedule=btk.SCHEDULE_QUEUE), off, d[0], d[1], d[2])
+    out_r = torch.zeros(n, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        out_r.zero_()
+        plan.execute(d[3], out_r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_r.zero_()
+        plan.execute(d[3], out_r)
+    torch.cuda.synchronize()
+    t_r = (time.perf_counter() - t0) / args.steps
+    plan.close()
+    replay_vs_solve = float(torch.linalg.norm(out_r - out_m) / torch.linalg.norm(out_m))
     t_1, out_1 = one_by_one(max(1, args.steps // 2), 1)
     # accuracy: 16 problems, up to 1000 sampled targets each, FP64 direct sum over the problem's sources
     g = np.random.default_rng(SEED + 1)
@@ -531,11 +554,17 @@ def bench_forest(args, btk, torch, wl, dev):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_m * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "name": args.workload, "problems": nprob, "n": n, "seed": SEED,
-                   "parallelism": "single GPU", "variant": "forest, mixed (FP32 kernel, FP64 accumulation)"},
+                   "parallelism": "single GPU",
+                   "variant": "forest, mixed (FP32 kernel, FP64 accumulation), persistent work-queue interactions"},
         "rel_l2": (num_m / den) ** 0.5, "rel_l2_plain_forest": (num_p / den) ** 0.5,
         "rel_l2_one_by_one": (num_1 / den) ** 0.5, "rel_l2_targets": "16 problems x <=1000 targets",
         "plain_forest_vs_one_by_one_rel": same,
         "ms_plain_forest": t_p * 1e3, "ms_one_by_one_fp64": t_1 * 1e3,
+        "ms_forest_group_launches": t_g * 1e3, "queue_over_group_launches": t_g / t_m,
+        "queue_vs_group_launches_rel": float(torch.linalg.norm(out_m - out_g) / torch.linalg.norm(out_g)),
+        "ms_plan_replay_queue": t_r * 1e3, "plan_replay_vs_solve_rel": replay_vs_solve,
+        "plan_replay_note": ("btk_plan_create_batch: trees, list and work items built once; each replay is one HIP "
+                             "graph (gather, upward, persistent interactions, downward) for the same charges"),
         "forest_fp64_over_one_by_one": t_1 / t_p, "forest_mixed_over_one_by_one": t_1 / t_m,
         "phases_s": {k: avg(st_m, "t_" + k) for k in ("tree", "traversal", "upward", "interact", "downward",
                                                       "total")},
@@ -548,10 +577,15 @@ def bench_forest(args, btk, torch, wl, dev):
     }
     t_near = avg(st_m, "t_kernel_near")
     if t_near > 0:
-        ach = (evals[0] + evals[1]) * FLOPS[(3, 0)] / t_near / 1e12
+        # the queue launch runs every PP/PC/CP/CC item; the shared-matrix CC
+        # products (t_kernel_far) are their own launches
+        nq = evals[0] + evals[1] + evals[2] + evals[3] * (1 - st_m[-1].cc_shared / max(1, st_m[-1].count[3]))
+        ach = nq * FLOPS[(3, 0)] / t_near / 1e12
         line["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["f32"],
                             "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS["f32"], 4), "traffic": None,
-                            "kernel": "k_interact near (PP+PC), FP32", "avg_launch_s": t_near}
+                            "kernel": "k_interact_queue (persistent: PP, PC, CP and unshared CC items), FP32",
+                            "avg_launch_s": t_near, "flops_per_launch": nq * FLOPS[(3, 0)],
+                            "flops_convention": FLOPS_NOTE}
     print(json.dumps(line), flush=True)
 
 
@@ -611,7 +645,8 @@ def main():
     res = {}
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
-        opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"))
+        opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"),
+                           schedule=btk.SCHEDULE_QUEUE if args.schedule == "queue" else btk.SCHEDULE_GROUPS)
         train = choice = None
         budget = None
         if v == "learned":  # untimed, like the warmup

@@ -96,6 +96,13 @@ enum {
     BTK_PREC_MIXED = 1  /* FP32 kernel evaluations, FP64 geometry, weights and per-target sums */
 };
 
+enum {
+    BTK_SCHEDULE_GROUPS = 0, /* one launch per group of work items (near or far, one degree) */
+    BTK_SCHEDULE_QUEUE = 1   /* one persistent launch drains every group's work items from one queue, biggest
+                                first, by atomic ticket (no reference counterpart; the reference launches one
+                                parallel_for per interaction type, poisson_fmm.cpp:21-25) */
+};
+
 /* Choices the reference makes implicitly.  Zero-initialised = reference behaviour. */
 typedef struct btk_options {
     int kernel;       /* BTK_KERNEL_* */
@@ -104,7 +111,8 @@ typedef struct btk_options {
     int deepest_only; /* 1 = leaf_to_point on the deepest level only (downward_pass.cpp:26-28) */
     int precision;    /* BTK_PREC_* */
     int mac;          /* 0 = the reference's fixed theta MAC; 1 = the learned MAC (btk_mac_*) where it accepts */
-    int reserved[2];
+    int schedule;     /* BTK_SCHEDULE_* of the PP/PC/CP/CC work items */
+    int reserved;
 } btk_options;
 
 /* Number of output components of a kernel: 1, 3, or 1 + dim. */
@@ -365,6 +373,15 @@ void btk_plan_destroy(btk_plan *plan);
 int btk_solve_batch(const btk_run_config *cfg, const btk_options *opts, int nprob, const int *offsets,
                     const double *d_x, const double *d_y, const double *d_z, const double *d_q, double *d_out,
                     btk_solve_stats *stats, void *stream);
+
+/* A solve plan for a fixed forest of nprob problems (no reference
+ * counterpart): btk_solve_batch's forest, list and work items built once, then
+ * btk_plan_execute replays gather, upward pass, interactions and downward
+ * pass as one HIP graph, adding into d_out[C][offsets[nprob]].  With
+ * opts->schedule = BTK_SCHEDULE_QUEUE the interactions of every problem are
+ * one persistent launch. */
+int btk_plan_create_batch(const btk_run_config *cfg, const btk_options *opts, int nprob, const int *offsets,
+                          const double *d_x, const double *d_y, const double *d_z, void *stream, btk_plan **plan);
 
 #ifdef __cplusplus
 }
