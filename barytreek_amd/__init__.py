@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libbtk.so")
 KERNEL_POT, KERNEL_REF_VEL3D, KERNEL_POT_GRAD = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 SCHEDULE_GROUPS, SCHEDULE_QUEUE = 0, 1
+SUM_BLOCKED, SUM_KAHAN = 0, 1
 
 EXPORTED = [
     "btk_components", "btk_last_error", "btk_release_cached", "btk_set_tuning", "btk_get_tuning",
@@ -65,7 +66,7 @@ class TreeInfo(C.Structure):
 class Options(C.Structure):
     _fields_ = [("kernel", C.c_int), ("convention", C.c_int), ("exclude_self", C.c_int),
                 ("deepest_only", C.c_int), ("precision", C.c_int), ("mac", C.c_int), ("schedule", C.c_int),
-                ("reserved", C.c_int)]
+                ("summation", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -224,10 +225,11 @@ def make_config(dim, degree, thresh, theta=0.7, eps=-1.0) -> RunConfig:
 
 
 def options(kernel=KERNEL_POT, convention=0, exclude_self=0, deepest_only=0, precision=PREC_FP64, mac=0,
-            schedule=0) -> Options:
+            schedule=0, summation=0) -> Options:
     o = Options()
     o.kernel, o.convention, o.exclude_self = kernel, convention, exclude_self
     o.deepest_only, o.precision, o.mac, o.schedule = deepest_only, precision, mac, schedule
+    o.summation = summation
     return o
 
 

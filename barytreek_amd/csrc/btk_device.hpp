@@ -143,6 +143,53 @@ __device__ __forceinline__ double comp_scale(int c, double gfc) {
 // FP32.  The 2D potential accumulates q log2 R, rescaled by ln 2 in
 // comp_scale_f32.
 // ---------------------------------------------------------------------------
+// Kahan's compensated summation (Kahan, CACM 8(1), 1965): c carries the
+// rounding error of the running sum a, whose value is a - c.
+__device__ __forceinline__ void kahan_add(float &a, float &c, float term) {
+    const float y = term - c;
+    const float t = a + y;
+    c = (t - a) - y;
+    a = t;
+}
+
+// The FP32 kernel with Kahan-compensated sums (acc, comp per component).
+template <int DIM, int KERNEL, bool EXCL>
+__device__ __forceinline__ void eval_f32_kahan(float dx, float dy, float dz, float q, float eps2, float *acc,
+                                               float *comp) {
+    if constexpr (DIM == 3) {
+        const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const bool skip = EXCL && R == eps2;
+        if constexpr (KERNEL == BTK_KERNEL_POT) {
+            const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
+            kahan_add(acc[0], comp[0], q * inv);
+        } else if constexpr (KERNEL == BTK_KERNEL_REF_VEL3D) {
+            const float g = skip ? 0.0f : q * __builtin_amdgcn_rcpf(R);
+            kahan_add(acc[0], comp[0], dx * g);
+            kahan_add(acc[1], comp[1], dy * g);
+            kahan_add(acc[2], comp[2], dz * g);
+        } else {
+            const float inv = skip ? 0.0f : __builtin_amdgcn_rsqf(R);
+            const float qi = q * inv;
+            kahan_add(acc[0], comp[0], qi);
+            const float h = qi * inv * inv;
+            kahan_add(acc[1], comp[1], h * dx);
+            kahan_add(acc[2], comp[2], h * dy);
+            kahan_add(acc[3], comp[3], h * dz);
+        }
+    } else {
+        (void)dz;
+        const float R = fmaf(dx, dx, fmaf(dy, dy, eps2));
+        const bool skip = EXCL && R == eps2;
+        const float lg = skip ? 0.0f : __builtin_amdgcn_logf(R);
+        kahan_add(acc[0], comp[0], q * lg);
+        if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
+            const float g = skip ? 0.0f : q * __builtin_amdgcn_rcpf(R);
+            kahan_add(acc[1], comp[1], g * dx);
+            kahan_add(acc[2], comp[2], g * dy);
+        }
+    }
+}
+
 template <int DIM, int KERNEL, bool EXCL>
 __device__ __forceinline__ void eval_f32(float dx, float dy, float dz, float q, float eps2, float *acc) {
     if constexpr (DIM == 3) {

@@ -240,6 +240,7 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.n1 = n1;
             L.tpt = g.tpt;
             L.waves = g.waves;
+            L.kahan = p.o.summation == BTK_SUM_KAHAN ? 1 : 0;
             L.far = far;
             L.excl = p.o.exclude_self ? 1 : 0;
             L.f32 = p.o.precision == BTK_PREC_MIXED ? 1 : 0;
@@ -427,6 +428,9 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
     BTK_REQUIRE(n1 >= 2 && n1 <= kMaxN1, "interp_degree must be between 1 and 10");
     BTK_REQUIRE(o.precision == BTK_PREC_FP64 || o.precision == BTK_PREC_MIXED, "unknown precision");
     BTK_REQUIRE(o.schedule == BTK_SCHEDULE_GROUPS || o.schedule == BTK_SCHEDULE_QUEUE, "unknown schedule");
+    BTK_REQUIRE(o.summation == BTK_SUM_BLOCKED || o.summation == BTK_SUM_KAHAN, "unknown near-field summation");
+    BTK_REQUIRE(!(o.summation == BTK_SUM_KAHAN && o.schedule == BTK_SCHEDULE_QUEUE),
+                "Kahan-compensated near sums run with the group launches (schedule 0)");
     BTK_REQUIRE(!(dim == 2 && o.kernel == BTK_KERNEL_REF_VEL3D), "the reference vel kernel is 3D only");
     BTK_REQUIRE(l.ndeg >= 1 && l.ndeg <= 4 && l.deg[l.ndeg - 1] == cfg.interp_degree,
                 "the list was built for another interp_degree");
@@ -475,6 +479,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             const int tpt_set = far ? tu.far_tpt : tu.near_tpt;
             // default: two targets per lane where the kernel is short (2D), one in 3D
             g.tpt = !f32 ? 1 : (tpt_set ? tpt_set : ((dim == 3 || (far && P <= kLanes)) ? 1 : 2));
+            if (!far && o.summation == BTK_SUM_KAHAN) g.tpt = 1;  // the compensated kernel: one target per lane
             // default: the 3D near launch shares each source tile between 4
             // wavefronts (a leaf holds hundreds of targets; 16M Plummer
             // points: 3.5x fewer bytes fetched at the same time); in 2D the
@@ -487,7 +492,7 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             const int per = kLanes * g.tpt * g.waves;
             // mixed precision, one tree, when asked for: mutual PP pairs once
             // for both directions (PP entries are in the degree-0 near group)
-            g.mutual = !far && di == 0 && f32 && &tt == &ts && tu.mutual;
+            g.mutual = !far && di == 0 && f32 && &tt == &ts && tu.mutual && o.summation != BTK_SUM_KAHAN;
             if (g.mutual) build_mutual(l, tt, g.tpt, p->meta.p, g.mu, s);
             // mixed precision, 3D: CC pairs that share a kernel matrix run as
             // MFMA products

@@ -225,7 +225,7 @@ using TileOf = std::conditional_t<F32 && FAR, TileRel, std::conditional_t<F32 &&
 
 // One work item, evaluated by the whole workgroup (64 * WAVES threads)
 // through the LDS tile `tile` of 64 * WAVES sources.
-template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES>
+template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES, bool KAHAN = false>
 __device__ __forceinline__ void interact_item(int item, const TaskView &tk, const EntryView &ev, const TgtView &tv,
                                               const SrcView &sv, int n1, const NodeTables &tabs, const KParams &kp,
                                               double *__restrict__ out, size_t comp_stride, int atomic,
@@ -304,20 +304,25 @@ __device__ __forceinline__ void interact_item(int item, const TaskView &tk, cons
             return load_source<DIM, F32>(mm, j, n1, tabs, sv);
         }
     };
-    auto eval = [&](const Tile &s, int k, float *a32, auto excl, auto relc) {
+    static_assert(!KAHAN || (F32 && !FAR), "compensated sums are a variant of the mixed-precision near kernel");
+    auto eval = [&](const Tile &s, int k, float *a32, float *c32, auto excl, auto relc) {
         constexpr bool X = decltype(excl)::value;
+        auto f32 = [&](float dx, float dy, float dz) {
+            if constexpr (KAHAN)
+                eval_f32_kahan<DIM, KERNEL, X>(dx, dy, dz, s.q, (float)kp.eps2, a32, c32);
+            else
+                eval_f32<DIM, KERNEL, X>(dx, dy, dz, s.q, (float)kp.eps2, a32);
+        };
+        (void)c32;
         if constexpr (REL)
             eval_pair_rel<DIM, KERNEL, X>(s, tr[k], (float)kp.eps2, a32);
         else if constexpr (MIX) {
             if constexpr (decltype(relc)::value)
-                eval_f32<DIM, KERNEL, X>(tr[k][0] - s.rx, tr[k][1] - s.ry, DIM == 3 ? tr[k][2] - s.rz : 0.0f, s.q,
-                                         (float)kp.eps2, a32);
+                f32(tr[k][0] - s.rx, tr[k][1] - s.ry, DIM == 3 ? tr[k][2] - s.rz : 0.0f);
             else if constexpr (DIM == 3)
-                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), (float)(t[k][2] - s.z),
-                                         s.q, (float)kp.eps2, a32);
+                f32((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), (float)(t[k][2] - s.z));
             else
-                eval_f32<DIM, KERNEL, X>((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f, s.q,
-                                         (float)kp.eps2, a32);
+                f32((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f);
         } else {
             static_assert(!F32, "mixed precision takes TileRel (far) or TileMix (near)");
             eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k]);
@@ -358,30 +363,31 @@ __device__ __forceinline__ void interact_item(int item, const TaskView &tk, cons
         // go into the FP64 accumulators, so the FP32 blocking stays at 64
         // terms whatever the tile size.
         auto run_chunk = [&](int c0, int cn, auto excl, auto relc) {
-            float a32[TPT][C];
+            float a32[TPT][C], c32[TPT][C];
 #pragma unroll
             for (int k = 0; k < TPT; k++)
 #pragma unroll
-                for (int c = 0; c < C; c++) a32[k][c] = 0.0f;
+                for (int c = 0; c < C; c++) a32[k][c] = c32[k][c] = 0.0f;
             if (cn == kLanes) {
 #pragma unroll 8
                 for (int jj = 0; jj < kLanes; jj++) {
                     const Tile s = tile[c0 + jj];
 #pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], excl, relc);
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], c32[k], excl, relc);
                 }
             } else {
                 for (int jj = 0; jj < cn; jj++) {
                     const Tile s = tile[c0 + jj];
 #pragma unroll
-                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], excl, relc);
+                    for (int k = 0; k < TPT; k++) eval(s, k, a32[k], c32[k], excl, relc);
                 }
             }
             if constexpr (F32) {
 #pragma unroll
                 for (int k = 0; k < TPT; k++)
 #pragma unroll
-                    for (int c = 0; c < C; c++) acc[k][c] += (double)a32[k][c];
+                    for (int c = 0; c < C; c++)
+                        acc[k][c] += KAHAN ? (double)a32[k][c] - (double)c32[k][c] : (double)a32[k][c];
             }
         };
         auto run_tile = [&](auto relc) {
@@ -432,13 +438,14 @@ __device__ __forceinline__ void interact_item(int item, const TaskView &tk, cons
 // The mixed-precision kernels ask for at least 6 wavefronts per SIMD (at most
 // 80 VGPRs); left to itself the compiler spends up to 113 on unrolling and
 // runs 4.  The FP64 kernels keep the compiler's choice.
-template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES>
+template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES, bool KAHAN = false>
 __global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
     k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     __shared__ TileOf<DIM, F32, FAR> tile[kLanes * WAVES];
-    interact_item<DIM, KERNEL, F32, TPT, FAR, EXCL, WAVES>(xcd_contiguous(blockIdx.x, gridDim.x), tk, ev, tv, sv, n1,
-                                                           tabs, kp, out, comp_stride, atomic, same_tree, tile);
+    interact_item<DIM, KERNEL, F32, TPT, FAR, EXCL, WAVES, KAHAN>(xcd_contiguous(blockIdx.x, gridDim.x), tk, ev, tv,
+                                                                  sv, n1, tabs, kp, out, comp_stride, atomic,
+                                                                  same_tree, tile);
 }
 
 // One group (near or far, one degree) of a work queue.  Its source proxy
@@ -592,6 +599,7 @@ struct InteractLaunch {
     TgtView tv;
     SrcView sv;
     int n_tasks, n1, tpt, waves, far, excl, f32, atomic;
+    int kahan;  // mixed-precision near launch: Kahan-compensated FP32 sums (one target per lane)
     int same_tree;  // targets and sources are one tree (the far group never compares panels)
     NodeTables tabs;
     KParams kp;
@@ -641,10 +649,25 @@ namespace btk {
         else BTK_IK_FE(F32_, TPT_, 1);                  \
     } while (0)
 
+#define BTK_IKK(EXCL_, W_)                                                                                  \
+    k_interact<DIM, KERNEL, true, 1, false, EXCL_, W_, true><<<L.n_tasks, kLanes * W_, 0, s>>>(              \
+        L.tk, L.ev, L.tv, L.sv, L.n1, L.tabs, L.kp, L.out, L.comp_stride, L.atomic, L.same_tree)
+#define BTK_IKK_W(EXCL_)                                \
+    do {                                                \
+        if (L.waves == 4) BTK_IKK(EXCL_, 4);            \
+        else if (L.waves == 2) BTK_IKK(EXCL_, 2);       \
+        else BTK_IKK(EXCL_, 1);                         \
+    } while (0)
+
 template <int DIM, int KERNEL>
 void launch_interact(const InteractLaunch &L, hipStream_t s) {
     if (L.n_tasks <= 0) return;
-    if (!L.f32)
+    if (L.kahan && L.f32 && !L.far) {
+        if (L.excl)
+            BTK_IKK_W(true);
+        else
+            BTK_IKK_W(false);
+    } else if (!L.f32)
         BTK_IK_W(false, 1);
     else if (L.tpt == 2)
         BTK_IK_W(true, 2);
@@ -653,6 +676,8 @@ void launch_interact(const InteractLaunch &L, hipStream_t s) {
     BTK_LAUNCH_CHECK();
 }
 
+#undef BTK_IKK_W
+#undef BTK_IKK
 #undef BTK_IK_W
 #undef BTK_IK_FE
 #undef BTK_IK

@@ -97,6 +97,11 @@ enum {
 };
 
 enum {
+    BTK_SUM_BLOCKED = 0, /* FP32 partial sums over blocks of 64 sources, each added into FP64 per target */
+    BTK_SUM_KAHAN = 1    /* the same blocks, each summed with Kahan's compensation (one target per lane) */
+};
+
+enum {
     BTK_SCHEDULE_GROUPS = 0, /* one launch per group of work items (near or far, one degree) */
     BTK_SCHEDULE_QUEUE = 1   /* one persistent launch drains every group's work items from one queue, biggest
                                 first, by atomic ticket (no reference counterpart; the reference launches one
@@ -112,7 +117,7 @@ typedef struct btk_options {
     int precision;    /* BTK_PREC_* */
     int mac;          /* 0 = the reference's fixed theta MAC; 1 = the learned MAC (btk_mac_*) where it accepts */
     int schedule;     /* BTK_SCHEDULE_* of the PP/PC/CP/CC work items */
-    int reserved;
+    int summation;    /* BTK_SUM_*: how mixed-precision PP/PC sums are kept in FP32 */
 } btk_options;
 
 /* Number of output components of a kernel: 1, 3, or 1 + dim. */
