@@ -110,6 +110,8 @@ def parse():
     ap.add_argument("--mac-sets", type=int, default=10, help="learned MAC: Plummer sets sampled for training")
     ap.add_argument("--mac-pairs", type=int, default=100_000, help="learned MAC: cluster pairs sampled per set")
     ap.add_argument("--mac-epochs", type=int, default=20, help="learned MAC: training epochs")
+    ap.add_argument("--summation", choices=["blocked", "kahan"], default="blocked",
+                    help="mixed-precision PP/PC sums: FP32 blocks of 64 sources, or the same with Kahan compensation")
     ap.add_argument("--schedule", choices=["groups", "queue"], default="groups",
                     help="interaction work items: one launch per group, or one persistent work-queue launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -646,7 +648,8 @@ def main():
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
         opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"),
-                           schedule=btk.SCHEDULE_QUEUE if args.schedule == "queue" else btk.SCHEDULE_GROUPS)
+                           schedule=btk.SCHEDULE_QUEUE if args.schedule == "queue" else btk.SCHEDULE_GROUPS,
+                           summation=btk.SUM_KAHAN if args.summation == "kahan" else btk.SUM_BLOCKED)
         train = choice = None
         budget = None
         if v == "learned":  # untimed, like the warmup
