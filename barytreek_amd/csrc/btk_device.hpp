@@ -72,6 +72,49 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
     return y;
 }
 
+// 1/x in FP64: hardware estimate refined by two Newton steps (about one unit
+// in the last place; the IEEE division costs three times the instructions).
+__device__ __forceinline__ double recip_f64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(fma(-x, y, 1.0), y, y);
+    y = fma(fma(-x, y, 1.0), y, y);
+    return y;
+}
+
+// log(x) in FP64 for finite x > 0 (a few units in the last place).
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)), so log x = e ln 2 + log m and
+// log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 3 - 2 sqrt(2) < 0.1716;
+// the Taylor series 2 (s + s^3/3 + ... + s^19/19) stops where the next term
+// is below 2^-54 s.  ln 2 = kLn2Hi + kLn2Lo with kLn2Hi's low 21 bits zero,
+// so e * kLn2Hi is exact for any exponent.
+constexpr double kLn2Hi = 0.6931471803691238;      // 0x3fe62e42fee00000
+constexpr double kLn2Lo = 1.9082149292705877e-10;  // ln 2 - kLn2Hi, rounded
+__device__ __forceinline__ double log_f64(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752440) {
+        m = m + m;
+        e -= 1;
+    }
+    const double f = m - 1.0;  // exact
+    const double s = f * recip_f64(m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double s2 = s + s;
+    const double lm = fma(s2 * z, p, s2);  // 2s + 2s z p
+    const double de = (double)e;
+    const double r = fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
+    return x > 0.0 ? r : -__builtin_inf();  // log 0 = -inf, as the C library's
+}
+
 // ---------------------------------------------------------------------------
 // Kernel evaluations.  acc[] collects the sums without the constant factor;
 // finish() applies it.  Components:
@@ -96,7 +139,7 @@ __device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double
             const double inv = skip ? 0.0 : rsqrt_f64(R);
             acc[0] = fma(q, inv, acc[0]);
         } else if constexpr (KERNEL == BTK_KERNEL_REF_VEL3D) {
-            const double g = skip ? 0.0 : q / R;
+            const double g = skip ? 0.0 : q * recip_f64(R);
             acc[0] = fma(dx, g, acc[0]);
             acc[1] = fma(dy, g, acc[1]);
             acc[2] = fma(dz, g, acc[2]);
@@ -113,10 +156,10 @@ __device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double
         (void)dz;
         const double R = fma(dx, dx, fma(dy, dy, eps2));
         const bool skip = EXCL && R == eps2;
-        const double lg = skip ? 0.0 : log(R);
+        const double lg = skip ? 0.0 : log_f64(R);
         acc[0] = fma(q, lg, acc[0]);
         if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
-            const double g = skip ? 0.0 : q / R;
+            const double g = skip ? 0.0 : q * recip_f64(R);
             acc[1] = fma(g, dx, acc[1]);
             acc[2] = fma(g, dy, acc[2]);
         }
