@@ -27,7 +27,7 @@ SUM_BLOCKED, SUM_KAHAN = 0, 1
 
 EXPORTED = [
     "btk_components", "btk_last_error", "btk_release_cached", "btk_set_tuning", "btk_get_tuning",
-    "btk_autotune",
+    "btk_autotune", "btk_autotune_multi",
     "btk_read_run_config",
     "btk_tree_construction_2d", "btk_tree_construction_3d", "btk_tree_get_info",
     "btk_tree_copy_panels_2d", "btk_tree_copy_panels_3d", "btk_tree_copy_point_leaf_panel",
@@ -72,7 +72,8 @@ class Options(C.Structure):
 class Tuning(C.Structure):
     """btk_tuning: launch shapes of the interaction kernels (0 = default)."""
     _fields_ = [("near_items", C.c_int), ("far_items", C.c_int), ("near_tpt", C.c_int), ("far_tpt", C.c_int),
-                ("mutual", C.c_int), ("cc_group", C.c_int), ("near_waves", C.c_int), ("far_waves", C.c_int)]
+                ("mutual", C.c_int), ("cc_group", C.c_int), ("near_waves", C.c_int), ("far_waves", C.c_int),
+                ("multi_vectors", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -122,6 +123,8 @@ def lib():
     L.btk_get_tuning.argtypes = [C.POINTER(Tuning)]
     L.btk_autotune.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int,
                                C.POINTER(Tuning), vp]
+    L.btk_autotune_multi.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int,
+                                     C.POINTER(Tuning), vp]
     L.btk_tree_construction_2d.argtypes = [cfgp, dp, dp, C.c_int, vp, C.POINTER(vp)]
     L.btk_tree_construction_3d.argtypes = [cfgp, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
     L.btk_tree_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
@@ -188,12 +191,14 @@ def get_tuning() -> Tuning:
 
 
 def set_tuning(near_items=0, far_items=0, near_tpt=0, far_tpt=0, mutual=0, cc_group=0, near_waves=0,
-               far_waves=0) -> None:
+               far_waves=0, multi_vectors=0) -> None:
     """Process-wide launch shapes of the interaction kernels; 0 = default
-    (cc_group: -1 turns the shared-matrix CC products off)."""
+    (cc_group: -1 turns the shared-matrix CC products off; multi_vectors:
+    charge vectors per workgroup of the many-vector MFMA kernels)."""
     t = Tuning()
     t.near_items, t.far_items, t.near_tpt, t.far_tpt = near_items, far_items, near_tpt, far_tpt
     t.mutual, t.cc_group, t.near_waves, t.far_waves = mutual, cc_group, near_waves, far_waves
+    t.multi_vectors = multi_vectors
     _check(lib().btk_set_tuning(C.byref(t)))
 
 
@@ -436,6 +441,20 @@ def solve(cfg, opts, xt, yt, zt, q, out, xs=None, ys=None, zs=None, stream=None)
                            int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), int(coincident), _ptr(out),
                            C.byref(st), _stream(stream)))
     return st
+
+
+def autotune_multi(cfg, opts, xt, yt, zt, q, xs=None, ys=None, zs=None, stream=None) -> Tuning:
+    """Times the many-vector MFMA launches of this problem (q is [K][n]) with
+    32, 64 and 128 charge vectors per workgroup and installs the fastest
+    process-wide (btk_autotune_multi)."""
+    coincident = xs is None
+    if coincident:
+        xs, ys, zs = xt, yt, zt
+    t = Tuning()
+    _check(lib().btk_autotune_multi(C.byref(cfg), C.byref(opts), int(xt.numel()), _ptr(xt), _ptr(yt), _ptr(zt),
+                                    int(xs.numel()), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), int(q.shape[0]),
+                                    int(coincident), C.byref(t), _stream(stream)))
+    return t
 
 
 def solve_multi(cfg, opts, xt, yt, zt, q, out, xs=None, ys=None, zs=None, stream=None) -> SolveStats:

@@ -146,6 +146,9 @@ int btk_set_tuning(const btk_tuning *t) {
         auto waves_ok = [](int w) { return w == 0 || w == 1 || w == 2 || w == 4; };
         BTK_REQUIRE(waves_ok(t->near_waves) && waves_ok(t->far_waves),
                     "wavefronts per workgroup must be 0 (default), 1, 2 or 4");
+        BTK_REQUIRE(t->multi_vectors == 0 || t->multi_vectors == 32 || t->multi_vectors == 64 ||
+                        t->multi_vectors == 128,
+                    "charge vectors per workgroup must be 0 (default), 32, 64 or 128");
         btk::tuning() = *t;
     });
 }
@@ -672,6 +675,7 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
                     t.far_tpt = b.tpt;
                     t.far_waves = b.waves;
                     t.cc_group = saved.cc_group;
+                    t.multi_vectors = saved.multi_vectors;
                     btk::tuning() = t;
                     kev.near_done = kev.far_done = false;
                     const auto c0 = std::chrono::steady_clock::now();
@@ -715,7 +719,97 @@ int btk_autotune(const btk_run_config *cfg_in, const btk_options *opts, int m, c
             best.far_waves = fc[jf].waves;
         }
         best.cc_group = saved.cc_group;
+        best.multi_vectors = saved.multi_vectors;
         if (o.precision != BTK_PREC_MIXED) best.near_tpt = best.far_tpt = 0;
+        btk::tuning() = best;
+        if (chosen) *chosen = best;
+    });
+}
+
+int btk_autotune_multi(const btk_run_config *cfg_in, const btk_options *opts, int m, const double *d_xt,
+                       const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                       const double *d_zs, const double *d_q, int nrhs, int coincident, btk_tuning *chosen,
+                       void *stream) {
+    return guarded([&] {
+        BTK_REQUIRE(cfg_in, "null run config");
+        btk_run_config cfg = *cfg_in;
+        const int dim = cfg.dim;
+        BTK_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        const btk_options o = opts_or_default(opts);
+        BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "btk_autotune_multi: potential or potential+gradient kernels");
+        BTK_REQUIRE(o.precision == BTK_PREC_MIXED && nrhs >= 16,
+                    "btk_autotune_multi: the MFMA path needs mixed precision and at least 16 charge vectors");
+        BTK_REQUIRE(!coincident || m == n, "coincident targets and sources must have the same count");
+        hipStream_t s = as_stream(stream);
+        auto ts = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xs, d_ys, d_zs, n, s);
+        std::unique_ptr<btk::Tree> tt_own;
+        if (!coincident) tt_own = btk::build_tree(dim, cfg.fmm_cluster_thresh, d_xt, d_yt, d_zt, m, s);
+        const btk::Tree &tt = coincident ? *ts : *tt_own;
+        auto l = btk::build_list(&cfg, tt, *ts, s, o.mac != 0);
+        BTK_REQUIRE(l->ndeg == 1, "btk_autotune_multi: one interpolation degree per list");
+        const int C = btk::components(dim, o.kernel);
+        const size_t wrows = btk::proxy_offset(*l, ts->pc, 1, l->ndeg), prow = btk::proxy_offset(*l, tt.pc, C, l->ndeg);
+        const int kp = (nrhs + 63) / 64 * 64;
+        btk::DBuf<float> qs((size_t)(n ? n : 1) * kp), wt(wrows * kp);
+        {
+            btk::DBuf<double> q(n ? n : 1), W((size_t)nrhs * wrows);
+            for (int r = 0; r < nrhs; r++) {
+                btk::gather_sorted(*ts, d_q + (size_t)r * n, q.p, s);
+                btk::upward_all(cfg, *l, *ts, q.p, W.p + (size_t)r * wrows, s);
+            }
+            btk::gather_multi_f32(*ts, d_q, nrhs, kp, qs.p, s);
+            btk::transpose_weights_f32((long long)wrows, nrhs, kp, W.p, wt.p, s);
+            BTK_HIP(hipStreamSynchronize(s));
+        }
+        btk::DBuf<double> Psi((size_t)nrhs * prow), out((size_t)nrhs * C * (m ? m : 1));
+        const int cand[] = {32, 64, 128};
+        const int ncand = 3;
+        const btk_tuning saved = btk::tuning();
+        btk::KernelEvents kev;
+        for (auto &e : kev.ev) BTK_HIP(hipEventCreate(&e));
+        // best near + far time of each candidate over `passes` passes (3 when
+        // the interactions take under 50 ms, 2 under 500 ms, else 1); odd
+        // passes run the candidates in reverse order
+        float best_ms[ncand];
+        for (float &b : best_ms) b = 1e30f;
+        int passes = 1;
+        try {
+            for (int pass = 0; pass < passes; pass++) {
+                for (int k = pass == 0 ? -1 : 0; k < ncand; k++) {  // k = -1 warms up
+                    const int i = (pass & 1) ? ncand - 1 - k : k;
+                    btk_tuning t = saved;
+                    t.multi_vectors = cand[std::max(i, 0)];
+                    btk::tuning() = t;
+                    kev.near_done = kev.far_done = false;
+                    const auto c0 = std::chrono::steady_clock::now();
+                    btk::fmm_interactions_multi(cfg, o, tt, *ts, *l, qs.p, wt.p, nrhs, kp, out.p, Psi.p, s, &kev);
+                    BTK_HIP(hipStreamSynchronize(s));
+                    if (i < 0) {
+                        const double sec =
+                            std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+                        passes = sec < 0.05 ? 3 : (sec < 0.5 ? 2 : 1);
+                        continue;
+                    }
+                    float total = 0.f, ms = 0.f;
+                    if (kev.near_done) {
+                        BTK_HIP(hipEventElapsedTime(&ms, kev.ev[0], kev.ev[1]));
+                        total += ms;
+                    }
+                    if (kev.far_done) {
+                        BTK_HIP(hipEventElapsedTime(&ms, kev.ev[2], kev.ev[3]));
+                        total += ms;
+                    }
+                    best_ms[i] = std::min(best_ms[i], total);
+                }
+            }
+        } catch (...) {
+            btk::tuning() = saved;
+            for (auto &e : kev.ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto &e : kev.ev) (void)hipEventDestroy(e);
+        btk_tuning best = saved;
+        best.multi_vectors = cand[std::min_element(best_ms, best_ms + ncand) - best_ms];
         btk::tuning() = best;
         if (chosen) *chosen = best;
     });

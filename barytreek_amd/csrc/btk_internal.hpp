@@ -190,6 +190,8 @@ int components(int dim, int kernel);
 
 // Process-wide launch shapes of the interaction kernels (zero = default).
 btk_tuning &tuning();
+// charge vectors per workgroup of the many-vector MFMA kernels (tuning, default 64)
+inline int multi_vectors() { return tuning().multi_vectors ? tuning().multi_vectors : 64; }
 
 // Phases.
 std::unique_ptr<Tree> build_tree(int dim, int thresh, const double *x, const double *y, const double *z, int n,

@@ -6,8 +6,9 @@
 // Tree, list, basis matrices and kernel evaluations are shared by all K
 // vectors, so the interactions become products G Q of the kernel tile G
 // (targets x sources) with the charge tile Q (sources x K):
-//   a workgroup of 4 wavefronts owns 64 targets of one panel and 64 charge
-//   vectors; wave w takes targets 32*(w&1) .. +32 and vectors 32*(w>>1) .. +32;
+//   a workgroup of KC/16 wavefronts owns 64 targets of one panel and KC
+//   charge vectors (KC = 32, 64 or 128, btk_tuning.multi_vectors); wave w
+//   takes targets 32*(w&1) .. +32 and vectors 32*(w>>1) .. +32;
 //   per 64-source LDS tile every lane evaluates G for its target and one
 //   source parity (FP64 difference rounded to FP32, kernel in FP32), which is
 //   exactly the A operand of v_mfma_f32_32x32x2f32, while the B operand is
@@ -22,8 +23,10 @@ namespace btk {
 namespace {
 
 constexpr int TB = 256;
-constexpr int KC = 64;  // charge vectors per workgroup
 inline int nblk(long long n) { return (int)((n + TB - 1) / TB); }
+// threads of a many-vector workgroup of KC charge vectors: two wavefronts
+// (64 targets) per 32 vectors, KC / 16 wavefronts in all
+constexpr int mrhs_threads(int kc) { return 4 * kc; }
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -63,11 +66,14 @@ struct MView {
 
 // Occupancy asked of the compiler: 5 wavefronts per SIMD for one component
 // (96 VGPRs, no spills; 4 by its own choice), 2 for three or four (1).
-template <int DIM, int KERNEL, bool FAR, bool EXCL>
-__global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2))
+template <int DIM, int KERNEL, bool FAR, bool EXCL, int KC>
+__global__ void __launch_bounds__(mrhs_threads(KC), (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2))
     k_interact_mrhs(TaskView tk, GroupView gv, TgtView tv, SrcView sv, MView mv, int n1, NodeTables tabs,
                     KParams kp, int M, int pct, int atomic, int same_tree) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
+    constexpr int NT = mrhs_threads(KC);
+    static_assert(KC == 32 || KC == 64 || KC == 128, "32, 64 or 128 charge vectors per workgroup");
+    static_assert(NT == 64 * 2 * (KC / 32), "wave w: targets 32 * (w & 1), vectors 32 * (w >> 1)");
     __shared__ double sc[3][64];
     __shared__ float scr[3][64];  // relative to the target box centre (separated boxes)
     __shared__ float qt[64][KC + 1];
@@ -78,6 +84,9 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
     const int first = tk.first[item];
     const int e0 = tk.e0[item], e1 = tk.e1[item];
     const int rhs0 = blockIdx.y * KC;
+    if (rhs0 >= mv.K) return;  // padding vectors only (the whole workgroup)
+    // a wavefront whose 32 vectors are all padding keeps to the barriers only
+    const bool idle = rhs0 + 32 * (w >> 1) >= mv.K;
     const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
@@ -154,9 +163,11 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
                 }
             }
             const float *qsrc = cluster ? mv.w + ((size_t)S * P + t0) * mv.kp : mv.q + (size_t)(ss + t0) * mv.kp;
-            for (int k = tid; k < 64 * KC; k += TB) {
+            // columns past the padded vector count (kp is a multiple of 64,
+            // the last block of 128 may reach past it) read as zero
+            for (int k = tid; k < 64 * KC; k += NT) {
                 const int s = k / KC, r = k % KC;
-                qt[s][r] = (s < nt) ? qsrc[(size_t)s * mv.kp + rhs0 + r] : 0.0f;
+                qt[s][r] = (s < nt && rhs0 + r < mv.kp) ? qsrc[(size_t)s * mv.kp + rhs0 + r] : 0.0f;
             }
             __syncthreads();
             f32x16 d[C];
@@ -190,10 +201,12 @@ __global__ void __launch_bounds__(TB, (KernelTraits<DIM, KERNEL>::C == 1 ? 5 : 2
                 for (int c = 0; c < C; c++) d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(g[c], b, d[c], 0, 0, 0);
             }
             };
-            if (rel)
-                steps(std::true_type{});
-            else
-                steps(std::false_type{});
+            if (!idle) {
+                if (rel)
+                    steps(std::true_type{});
+                else
+                    steps(std::false_type{});
+            }
 #pragma unroll
             for (int c = 0; c < C; c++)
 #pragma unroll
@@ -557,27 +570,51 @@ void transpose_weights_f32(long long rows, int K, int kp, const double *W, float
     BTK_LAUNCH_CHECK();
 }
 
+template <int DIM, int KERNEL, bool FAR, bool EXCL, int KC>
+void launch_mrhs_kc(const TaskView &tk, int n_tasks, const GroupView &gv, const TgtView &tv, const SrcView &sv,
+                    const MView &mv, int n1, const NodeTables &tabs, const KParams &kp_, int M, int pct, int atomic,
+                    int same_tree, hipStream_t s) {
+    const dim3 grid(n_tasks, (mv.kp + KC - 1) / KC);
+    k_interact_mrhs<DIM, KERNEL, FAR, EXCL, KC><<<grid, mrhs_threads(KC), 0, s>>>(tk, gv, tv, sv, mv, n1, tabs, kp_,
+                                                                                  M, pct, atomic, same_tree);
+}
+
+template <int DIM, int KERNEL, bool FAR, bool EXCL>
+void launch_mrhs_fe(int kc, const TaskView &tk, int n_tasks, const GroupView &gv, const TgtView &tv,
+                    const SrcView &sv, const MView &mv, int n1, const NodeTables &tabs, const KParams &kp_, int M,
+                    int pct, int atomic, int same_tree, hipStream_t s) {
+    if (kc == 32)
+        launch_mrhs_kc<DIM, KERNEL, FAR, EXCL, 32>(tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                   same_tree, s);
+    else if (kc == 128)
+        launch_mrhs_kc<DIM, KERNEL, FAR, EXCL, 128>(tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                    same_tree, s);
+    else
+        launch_mrhs_kc<DIM, KERNEL, FAR, EXCL, 64>(tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                   same_tree, s);
+}
+
 template <int DIM, int KERNEL>
 void launch_mrhs(const TaskView &tk, int n_tasks, const GroupView &gv, const TgtView &tv, const SrcView &sv,
                  const float *qs, const float *wt, double *out, int K, int kp, int n1, const NodeTables &tabs,
-                 const KParams &kp_, int M, int pct, int atomic, int same_tree, bool far, bool excl, hipStream_t s) {
+                 const KParams &kp_, int M, int pct, int atomic, int same_tree, bool far, bool excl, int kc,
+                 hipStream_t s) {
     if (n_tasks <= 0) return;
     const MView mv{qs, wt, out, K, kp};
-    const dim3 grid(n_tasks, kp / KC);
     if (far) {
         if (excl)
-            k_interact_mrhs<DIM, KERNEL, true, true><<<grid, TB, 0, s>>>(tk, gv, tv, sv, mv, n1, tabs, kp_, M, pct,
-                                                                        atomic, same_tree);
+            launch_mrhs_fe<DIM, KERNEL, true, true>(kc, tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                    same_tree, s);
         else
-            k_interact_mrhs<DIM, KERNEL, true, false><<<grid, TB, 0, s>>>(tk, gv, tv, sv, mv, n1, tabs, kp_, M, pct,
-                                                                         atomic, same_tree);
+            launch_mrhs_fe<DIM, KERNEL, true, false>(kc, tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                     same_tree, s);
     } else {
         if (excl)
-            k_interact_mrhs<DIM, KERNEL, false, true><<<grid, TB, 0, s>>>(tk, gv, tv, sv, mv, n1, tabs, kp_, M, pct,
-                                                                         atomic, same_tree);
+            launch_mrhs_fe<DIM, KERNEL, false, true>(kc, tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct, atomic,
+                                                     same_tree, s);
         else
-            k_interact_mrhs<DIM, KERNEL, false, false><<<grid, TB, 0, s>>>(tk, gv, tv, sv, mv, n1, tabs, kp_, M,
-                                                                          pct, atomic, same_tree);
+            launch_mrhs_fe<DIM, KERNEL, false, false>(kc, tk, n_tasks, gv, tv, sv, mv, n1, tabs, kp_, M, pct,
+                                                      atomic, same_tree, s);
     }
     BTK_LAUNCH_CHECK();
 }
@@ -595,20 +632,21 @@ void run_multi(const btk_run_config &cfg, const btk_options &o, const Tree &tt, 
     const TgtView tv{tt.start.p, tt.count.p, tt.perm.p, tt.sx.p, tt.sy.p, tt.sz.p,
                      tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p};
     const GroupView gv{l.gsrc.p, l.gtype.p};
-    const int chunks = kp / KC;
+    const int kc = multi_vectors();
+    const int chunks = (kp + kc - 1) / kc;
     for (int far = 0; far < 2; far++) {
         if (l.counts[2 * far] + l.counts[2 * far + 1] == 0) continue;
         const int n_clusters = tt.pc - tt.n_leaves;
-        // base counts wavefronts (workgroups of 4): aim at 8192 of them
+        // base counts wavefronts (kc / 16 per workgroup): aim at 8192 of them
         const long long base = (far ? (long long)n_clusters * ((P + 63) / 64) : (long long)tt.n / 64 + tt.n_leaves) *
-                               chunks * 4;
+                               chunks * (kc / 16);
         const int nsplit = choose_split(base, 8192);
         Tasks tk;
         build_tasks(l, tt, ts, far, P, 64, nsplit, tk, s);
         if (kev) BTK_HIP(hipEventRecord(kev->ev[far ? 2 : 0], s));
         launch_mrhs<DIM, KERNEL>(TaskView{tk.panel.p, tk.first.p, tk.e0.p, tk.e1.p}, tk.n, gv, tv, sv, qs, wt,
                                  far ? Psi : out, K, kp, n1, tabs, kp_, tt.n, tt.pc, nsplit > 1 ? 1 : 0,
-                                 (&tt == &ts) ? 1 : 0, far != 0, o.exclude_self != 0, s);
+                                 (&tt == &ts) ? 1 : 0, far != 0, o.exclude_self != 0, kc, s);
         if (kev) {
             BTK_HIP(hipEventRecord(kev->ev[far ? 3 : 1], s));
             (far ? kev->far_done : kev->near_done) = true;
@@ -620,7 +658,7 @@ void run_multi(const btk_run_config &cfg, const btk_options &o, const Tree &tt, 
 void fmm_interactions_multi(const btk_run_config &cfg, const btk_options &o, const Tree &tt, const Tree &ts,
                             const List &l, const float *qs, const float *wt, int K, int kp, double *out, double *Psi,
                             hipStream_t s, KernelEvents *kev) {
-    BTK_REQUIRE(kp % KC == 0 && kp >= K, "charge-vector padding must be a multiple of 64");
+    BTK_REQUIRE(kp % 64 == 0 && kp >= K, "charge-vector padding must be a multiple of 64");
     BTK_REQUIRE(o.kernel != BTK_KERNEL_REF_VEL3D, "many charge vectors: potential or potential+gradient kernels");
     BTK_REQUIRE(l.ndeg == 1, "many charge vectors: one interpolation degree per list");
     if (tt.dim == 3) {
@@ -639,19 +677,19 @@ void fmm_interactions_multi(const btk_run_config &cfg, const btk_options &o, con
 template void launch_mrhs<2, BTK_KERNEL_POT>(const TaskView &, int, const GroupView &, const TgtView &,
                                               const SrcView &, const float *, const float *, double *, int, int, int,
                                               const NodeTables &, const KParams &, int, int, int, int, bool, bool,
-                                              hipStream_t);
+                                              int, hipStream_t);
 template void launch_mrhs<2, BTK_KERNEL_POT_GRAD>(const TaskView &, int, const GroupView &, const TgtView &,
                                                    const SrcView &, const float *, const float *, double *, int, int,
                                                    int, const NodeTables &, const KParams &, int, int, int, int, bool,
-                                                   bool, hipStream_t);
+                                                   bool, int, hipStream_t);
 template void launch_mrhs<3, BTK_KERNEL_POT>(const TaskView &, int, const GroupView &, const TgtView &,
                                               const SrcView &, const float *, const float *, double *, int, int, int,
                                               const NodeTables &, const KParams &, int, int, int, int, bool, bool,
-                                              hipStream_t);
+                                              int, hipStream_t);
 template void launch_mrhs<3, BTK_KERNEL_POT_GRAD>(const TaskView &, int, const GroupView &, const TgtView &,
                                                    const SrcView &, const float *, const float *, double *, int, int,
                                                    int, const NodeTables &, const KParams &, int, int, int, int, bool,
-                                                   bool, hipStream_t);
+                                                   bool, int, hipStream_t);
 
 
 void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,

@@ -11,9 +11,10 @@ Three variants are timed on it: the plain FP64 treecode, the mixed-precision
 treecode with the fixed theta MAC, and the mixed-precision treecode with the
 learned MAC.  The learned MAC's network is trained first on cluster pairs
 sampled from the treecode's own runs on other Plummer sets, and its per-pair
-error budget is calibrated on held-out Plummer sets of the same size (other
-seeds) against their FP64 direct sums; the benchmarked input's own direct sum
-only judges the result afterwards.
+error budget is calibrated on three held-out Plummer sets of the same size
+(other seeds) against their FP64 direct sums: the largest budget whose worst
+held-out error, and mean + 2 standard deviations, stay within 8.5e-7.  The
+benchmarked input's own direct sum only judges the result afterwards.
 
 Prints one JSON line.  `value` is targets/s of the whole job for the fastest
 variant whose relative L2 error against the FP64 direct sum (every target up
@@ -25,7 +26,8 @@ the warmup.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant both|plain|mixed|learned]
                     [--workload 3d-plummer-16M|2d|3d-uniform-100k|3d-sphere-plummer-4M|
                                 3d-batched-2M|3d-forest-256]
-                    [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL] [--no-autotune]
+                    [--shapes NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES]]
+                    [--no-autotune] [--schedule groups|queue] [--summation blocked|kahan] [--multi-vectors V]
                     [--no-cpu-baseline] [--cpu-stride K] [--no-accuracy] [--no-degree-scan]
                     [--mac-budget B] [--mac-target E] [--mac-holdout H]
                     [--mac-sets S] [--mac-pairs P] [--mac-epochs E] [--n N] [--leaf L] [--degree D]
@@ -117,6 +119,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--multi-vectors", type=int, default=0, choices=[0, 32, 64, 128],
+                    help="3d-batched-2M: charge vectors per MFMA workgroup instead of autotuning them")
     ap.add_argument("--mac-budget", type=float, default=None,
                     help="learned MAC: this per-pair budget instead of the held-out calibration (for profiling)")
     ap.add_argument("--mac-target", type=float, default=0.85e-6,
@@ -425,6 +429,13 @@ def bench_batched(args, btk, torch, wl, dev):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, out.view(k, n), st
 
+    shape = None
+    if args.multi_vectors:
+        btk.set_tuning(multi_vectors=args.multi_vectors)
+    elif not args.no_autotune:
+        print("[bench] autotune: charge vectors per workgroup", file=sys.stderr, flush=True)
+        o = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=btk.PREC_MIXED)
+        shape = btk.autotune_multi(cfg, o, d[0], d[1], d[2], dq).multi_vectors
     t_mixed, out_m, st = timed(btk.PREC_MIXED, K, args.steps, args.warmup)
     kp = min(4, K)
     t_plain_k, out_p, _ = timed(btk.PREC_FP64, kp, 1, 1)
@@ -457,6 +468,8 @@ def bench_batched(args, btk, torch, wl, dev):
         "t_kernel_near_s": st.t_kernel_near, "t_kernel_far_s": st.t_kernel_far,
         "evals_per_vector": {"pp": evals[0], "pc": evals[1], "cp": evals[2], "cc": evals[3]},
         "cpu_baseline": None,
+        "launch_shapes": {"multi_vectors": btk.get_tuning().multi_vectors or 64,
+                          "autotuned": shape is not None},
     }
     # near kernel: 2 flops per (target, source, vector) product on MFMA
     prod = (evals[0] + evals[1]) * K * 2

@@ -149,6 +149,9 @@ typedef struct btk_tuning {
                        consecutive blocks of a panel's targets and share one LDS tile of 64 * near_waves
                        sources, so each source is read from memory once per workgroup */
     int far_waves;  /* the same for CP+CC (0 = default, 1) */
+    int multi_vectors; /* btk_solve_multi's MFMA kernels: charge vectors per workgroup, 32, 64 or 128
+                          (0 = default, 64); a workgroup of multi_vectors / 16 wavefronts takes 64 targets
+                          and evaluates each kernel value once per 32 vectors */
 } btk_tuning;
 int btk_set_tuning(const btk_tuning *t);
 int btk_get_tuning(btk_tuning *t);
@@ -163,6 +166,19 @@ int btk_get_tuning(btk_tuning *t);
 int btk_autotune(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt, const double *d_yt,
                  const double *d_zt, int n, const double *d_xs, const double *d_ys, const double *d_zs,
                  const double *d_q, int coincident, btk_tuning *chosen, void *stream);
+
+/* Autotuner of btk_solve_multi (no reference counterpart): with the arguments
+ * of btk_solve_multi (d_q holds nrhs charge vectors of n sources), builds the
+ * trees, list and proxy weights once, times the many-vector PP+PC and CP+CC
+ * launches with 32, 64 and 128 charge vectors per workgroup, installs the
+ * fastest (btk_tuning.multi_vectors; the other fields are kept) and returns
+ * the tuning in *chosen (may be null).  Needs mixed precision and nrhs >= 16,
+ * the case btk_solve_multi runs on MFMA.  Writes no caller memory besides
+ * *chosen. */
+int btk_autotune_multi(const btk_run_config *cfg, const btk_options *opts, int m, const double *d_xt,
+                       const double *d_yt, const double *d_zt, int n, const double *d_xs, const double *d_ys,
+                       const double *d_zs, const double *d_q, int nrhs, int coincident, btk_tuning *chosen,
+                       void *stream);
 
 /* read_run_config (read_namelist.hpp:7, read_namelist.cpp:10-44).
  * cfg->dim must be set before the call, as in the reference drivers. */

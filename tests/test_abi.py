@@ -37,7 +37,7 @@ def test_record_layouts_match_reference():
     assert ctypes.sizeof(Options) == 32
     assert ctypes.sizeof(SolveStats) == 6 * 8 + 4 * 8 + 4 * 8 + 2 * 8 + 4 * 4 + 8
     assert SolveStats.cc_shared.offset == 144
-    assert ctypes.sizeof(btk.Tuning) == 32
+    assert ctypes.sizeof(btk.Tuning) == 36
 
 
 def test_tuning_round_trip_and_rejects_bad_values():
@@ -56,6 +56,11 @@ def test_tuning_round_trip_and_rejects_bad_values():
         assert (t.near_waves, t.far_waves) == (4, 2)
         with pytest.raises(btk.BtkError, match="wavefronts per workgroup"):
             btk.set_tuning(near_waves=3)
+        for v in (32, 64, 128):
+            btk.set_tuning(multi_vectors=v)
+            assert btk.get_tuning().multi_vectors == v
+        with pytest.raises(btk.BtkError, match="charge vectors per workgroup"):
+            btk.set_tuning(multi_vectors=96)
     finally:
         btk.restore_tuning(old)
 

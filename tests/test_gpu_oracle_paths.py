@@ -134,8 +134,8 @@ MULTI = [
 @pytest.mark.parametrize("kind,n,dim,deg,leaf,theta,eps,kernel,excl,nrhs", MULTI)
 def test_many_vectors_vs_oracle(kind, n, dim, deg, leaf, theta, eps, kernel, excl, nrhs):
     """btk_solve_multi in mixed precision (FP32 MFMA products for >= 16
-    vectors): sampled vectors, each against the oracle's FP64 solve of that
-    vector alone."""
+    vectors), with 32, 64 and 128 charge vectors per workgroup: sampled
+    vectors, each against the oracle's FP64 solve of that vector alone."""
     print(f"\n[oracle paths] many vectors {kind} n={n} dim={dim} kernel={kernel} vectors={nrhs}", flush=True)
     O.set_threads(16)
     x, y, z, _ = points(kind, n, 40 + nrhs)
@@ -143,16 +143,24 @@ def test_many_vectors_vs_oracle(kind, n, dim, deg, leaf, theta, eps, kernel, exc
     cfg = btk.make_config(dim, deg, leaf, theta, eps)
     C = btk.components(dim, kernel)
     d = [None if a is None else torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (x, y, z)]
-    out = torch.zeros(nrhs * C * n, dtype=torch.float64, device="cuda")
-    btk.solve_multi(cfg, btk.options(kernel=kernel, exclude_self=excl, precision=btk.PREC_MIXED), d[0], d[1], d[2],
-                    torch.from_numpy(Q).cuda(), out)
-    got = out.view(nrhs, C, n).cpu().numpy()
+    got = {}
+    for mv in (32, 64, 128):
+        btk.set_tuning(multi_vectors=mv)
+        out = torch.zeros(nrhs * C * n, dtype=torch.float64, device="cuda")
+        btk.solve_multi(cfg, btk.options(kernel=kernel, exclude_self=excl, precision=btk.PREC_MIXED), d[0], d[1],
+                        d[2], torch.from_numpy(Q).cuda(), out)
+        got[mv] = out.view(nrhs, C, n).cpu().numpy()
     ocfg = O.make_config(dim, deg, leaf, theta, eps)
     for r in (0, nrhs // 2, nrhs - 1):
         ref, _, _ = O.solve(ocfg, O.opts(kernel=kernel, exclude_self=excl), x, y, z, x, y, z, Q[r])
-        e = rel(got[r], ref)
-        print(f"  vector {r}: vs oracle {e:.2e}", flush=True)
-        assert e < MIXED_TOL
+        for mv in (32, 64, 128):
+            e = rel(got[mv][r], ref)
+            print(f"  vector {r}, {mv} vectors per workgroup: vs oracle {e:.2e}", flush=True)
+            assert e < MIXED_TOL, mv
+    # a column's FP32 tile products are the same MFMA steps in every
+    # partition of the vectors; only the FP64 adds of the tiles (and of split
+    # items) may run in another order
+    assert rel(got[32], got[64]) < 1e-12 and rel(got[128], got[64]) < 1e-12
 
 
 @pytest.mark.parametrize("kernel,prec,deg", [(btk.KERNEL_POT_GRAD, btk.PREC_FP64, 6),

@@ -47,6 +47,24 @@ def test_autotune_installs_a_candidate(dim, prec):
         assert t.near_tpt == 0 and t.far_tpt == 0 and t.mutual == 0
 
 
+def test_autotune_multi_installs_a_candidate():
+    """btk_autotune_multi picks 32, 64 or 128 charge vectors per workgroup
+    and keeps every other field; the rest of the tuning is untouched."""
+    print("\n[tuning] autotune_multi", flush=True)
+    d = _problem(3, 20000, 5)
+    Q = torch.from_numpy(np.random.default_rng(5).standard_normal((40, 20000))).cuda()
+    cfg = btk.make_config(3, 6, 300, 0.7, 0.0)
+    o = btk.options(kernel=btk.KERNEL_POT, exclude_self=1, precision=btk.PREC_MIXED)
+    btk.set_tuning(near_items=16384, near_waves=2, cc_group=4)
+    t = btk.autotune_multi(cfg, o, d[0], d[1], d[2], Q)
+    g = btk.get_tuning()
+    print(f"  chosen: {t.multi_vectors} vectors per workgroup", flush=True)
+    assert t.multi_vectors in (32, 64, 128) and g.multi_vectors == t.multi_vectors
+    assert (g.near_items, g.near_waves, g.cc_group) == (16384, 2, 4)
+    with pytest.raises(btk.BtkError, match="mixed precision"):
+        btk.autotune_multi(cfg, btk.options(kernel=btk.KERNEL_POT, exclude_self=1), d[0], d[1], d[2], Q)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_results_independent_of_launch_shapes(dim):
     print(f"\n[tuning] shapes dim={dim}", flush=True)
