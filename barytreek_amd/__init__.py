@@ -40,6 +40,7 @@ EXPORTED = [
     "btk_direct_sum_2d", "btk_direct_sum_3d", "btk_solve", "btk_solve_multi", "btk_solve_batch",
     "btk_plan_create", "btk_plan_create_batch", "btk_plan_execute", "btk_plan_destroy",
     "btk_mac_sample", "btk_mac_train", "btk_mac_set_weights", "btk_mac_get_weights", "btk_mac_set_budget",
+    "btk_mac_set_costs",
     "btk_mac_forward", "btk_mac_gradients",
 ]
 
@@ -164,6 +165,7 @@ def lib():
     L.btk_mac_set_weights.argtypes = [C.POINTER(C.c_float), C.c_int]
     L.btk_mac_get_weights.argtypes = [C.POINTER(C.c_float), C.c_int]
     L.btk_mac_set_budget.argtypes = [C.c_double]
+    L.btk_mac_set_costs.argtypes = [C.POINTER(C.c_float)]
     L.btk_mac_forward.argtypes = [fp, C.c_int, fp, vp]
     L.btk_mac_gradients.argtypes = [fp, fp, C.c_int, fp, C.POINTER(C.c_float), vp]
     L.btk_plan_create.argtypes = [cfgp, optp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, vp, C.POINTER(vp)]
@@ -561,6 +563,13 @@ def mac_get_weights() -> np.ndarray:
 
 def mac_set_budget(rel_err: float) -> None:
     _check(lib().btk_mac_set_budget(float(rel_err)))
+
+
+def mac_set_costs(pp=1.0, pc=1.0, cp=1.0, cc=1.0) -> None:
+    """Relative cost of one PP, PC, CP and CC kernel evaluation for the
+    learned MAC's choice (btk_mac_set_costs; all 1 = fewest evaluations)."""
+    c = (C.c_float * 4)(pp, pc, cp, cc)
+    _check(lib().btk_mac_set_costs(c))
 
 
 def mac_forward(X, n, out, stream=None):

@@ -2,6 +2,7 @@
 // status code and a thread-local message.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -849,6 +850,15 @@ int btk_mac_set_budget(double rel_err) {
     return guarded([&] {
         BTK_REQUIRE(rel_err > 0 && rel_err < 1, "the budget is a relative error in (0, 1)");
         btk::mac_model().budget = rel_err;
+    });
+}
+
+int btk_mac_set_costs(const float *costs) {
+    return guarded([&] {
+        BTK_REQUIRE(costs, "null argument");
+        for (int k = 0; k < 4; k++)
+            BTK_REQUIRE(costs[k] > 0 && std::isfinite(costs[k]), "costs per evaluation must be positive");
+        for (int k = 0; k < 4; k++) btk::mac_model().cost[k] = costs[k];
     });
 }
 

@@ -264,6 +264,7 @@ struct MacModel {
     DBuf<float> w;  // BTK_MAC_WEIGHTS parameters
     bool ready = false;
     double budget = 1e-7;
+    float cost[4] = {1.0f, 1.0f, 1.0f, 1.0f};  // relative cost of one PP, PC, CP, CC evaluation
 };
 MacModel &mac_model();
 void mac_set_weights(const float *h, hipStream_t s);

@@ -426,6 +426,7 @@ MacView mac_view(const btk_run_config &cfg, const Tree &tt, const Tree &ts) {
     v.w = m.w.p;
     v.log_budget = (float)(std::log10(m.budget) / kMacScale);
     v.di = n / 2 - 2;
+    for (int k = 0; k < 4; k++) v.cost[k] = m.cost[k];
     v.T = MacPanels{tt.lo[0].p, tt.lo[1].p, tt.lo[2].p, tt.hi[0].p, tt.hi[1].p, tt.hi[2].p, tt.radius.p,
                     tt.count.p, tt.level.p};
     v.S = MacPanels{ts.lo[0].p, ts.lo[1].p, ts.lo[2].p, ts.hi[0].p, ts.hi[1].p, ts.hi[2].p, ts.radius.p,

@@ -96,6 +96,7 @@ struct MacView {
     const float *w = nullptr;  // null: the theta MAC alone
     float log_budget = 0.0f;   // log10(budget) / 4, compared with the output
     int di = 0;                // degree index of the configured degree (4, 6, 8, 10)
+    float cost[4] = {1.0f, 1.0f, 1.0f, 1.0f};  // relative cost of one PP, PC, CP, CC evaluation
     MacPanels T{}, S{};
 };
 
@@ -125,16 +126,18 @@ __device__ __forceinline__ bool mac_predict(int it, int is, const MacView &m, co
 // up to m.di whose predicted error is within the budget: CC for any two
 // panels, PC and CP where the panel on the point side holds at most thr
 // points (a leaf, as in the reference's types); the pair takes the candidate
-// with the fewest kernel evaluations.  A pair the theta MAC would refine is
-// accepted only by an approximation cheaper than its direct sum.
+// of least cost, its kernel evaluations weighted by the type's cost per
+// evaluation (m.cost; equal weights count evaluations).  A pair the theta
+// MAC would refine is accepted only by an approximation cheaper than its
+// direct sum.
 __device__ __forceinline__ int mac_decide(int it, int is, const MacView &m, const float *w, int thr, int type,
                                           bool accepted) {
     const int ct = m.T.count[it], cs = m.S.count[is];
     const double nt = ct, ns = cs;
-    const double direct = nt * ns;
+    const double direct = nt * ns * m.cost[0];
     auto cost = [&](int t, int d) {
         const double P = mac_proxies(d);
-        return t == 1 ? nt * P : (t == 2 ? P * ns : P * P);
+        return (t == 1 ? nt * P : (t == 2 ? P * ns : P * P)) * m.cost[t];
     };
     double best = direct;
     int code = -1;

@@ -85,11 +85,12 @@ __global__ void __launch_bounds__(mrhs_threads(KC), (KernelTraits<DIM, KERNEL>::
     const int e0 = tk.e0[item], e1 = tk.e1[item];
     const int rhs0 = blockIdx.y * KC;
     if (rhs0 >= mv.K) return;  // padding vectors only (the whole workgroup)
-    // a wavefront whose 32 vectors are all padding keeps to the barriers only
-    const bool idle = rhs0 + 32 * (w >> 1) >= mv.K;
     const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
     const int ntgt = FAR ? P : tv.count[T];
     const int ts0 = FAR ? 0 : tv.start[T];
+    // a wavefront whose 32 vectors or 32 targets are all padding keeps to the
+    // barriers only
+    const bool idle = rhs0 + 32 * (w >> 1) >= mv.K || first + 32 * (w & 1) >= ntgt;
     // this lane's target (A rows) and charge-vector column (B, D columns)
     const int ti = first + 32 * (w & 1) + (lane & 31);
     const int tc = min(ti, ntgt - 1);
@@ -173,9 +174,10 @@ __global__ void __launch_bounds__(mrhs_threads(KC), (KernelTraits<DIM, KERNEL>::
             f32x16 d[C];
 #pragma unroll
             for (int c = 0; c < C; c++) d[c] = (f32x16){0};
+            const int nsteps = (nt + 1) / 2;  // a partial last tile runs only its sources' steps
             auto steps = [&](auto relc) {
 #pragma unroll 4
-            for (int step = 0; step < 32; step++) {
+            for (int step = 0; step < nsteps; step++) {
                 const int s = 2 * step + par;
                 float dx, dy, dz;
                 if constexpr (decltype(relc)::value) {
@@ -304,11 +306,13 @@ __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct
         while (root >= kl.nprob) root = tparent[root];
         const long long half = 1ll << (kl.ob - 1);
         bool ok = lt < 63 && ls < 63;  // (a key of all ones would tie with kNoKey)
+        bool apart = false;  // separated boxes: no zero-distance proxy pair (k_cc_gemm has no such test)
         unsigned long long k = ((((unsigned long long)root << 6) | (unsigned long long)lt) << 6 | ls) << (3 * kl.ob);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double tl = sel3(d, T.lo0[t], T.lo1[t], T.lo2[t]), th = sel3(d, T.hi0[t], T.hi1[t], T.hi2[t]);
             const double sl = sel3(d, S.lo0[s], S.lo1[s], S.lo2[s]), sh = sel3(d, S.hi0[s], S.hi1[s], S.hi2[s]);
+            apart = apart || sl > th || tl > sh;
             const double h = 0.5 * fmin(th - tl, sh - sl);
             const double off = 0.5 * (sl + sh) - 0.5 * (tl + th);
             long long q = 0;
@@ -321,7 +325,7 @@ __global__ void k_cc_keys(int n, int e0, const int *__restrict__ goff_g, int pct
             }
             k |= (unsigned long long)(q + half) << (kl.ob * (2 - d));
         }
-        if (ok) key = k;
+        if (ok && apart) key = k;
     }
     keys[i] = key;
 }
@@ -403,9 +407,8 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
 }
 
 // One work item: 128 target proxies x up to 32 pairs of one key, components
-// C0 .. C0+NC-1 of the kernel (the gradient kernel runs as two launches of
-// two components, which halves the accumulators and lets two wavefronts
-// share a SIMD).  Wave w takes targets 32*w .. +32 and all 32 pairs and
+// C0 .. C0+NC-1 of the kernel (every component in one launch: each kernel
+// value feeds NC products).  Wave w takes targets 32*w .. +32 and all 32 pairs and
 // forms the transposed product, pairs x targets: the A operand holds the
 // pairs' proxy weights (FP32, from LDS, loaded one tile ahead into
 // registers), the B operand the kernel of the key's first pair (FP32
@@ -414,10 +417,11 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
 // FP32 tile products are added into FP64 sums; each accumulator row is one
 // pair, its lanes 32 consecutive target proxies, so the FP64 atomic adds
 // into each pair's own target cluster are coalesced.
-// At least 3 wavefronts per SIMD (155 VGPRs, no spills; 2 by the compiler's
-// own choice).
-template <int KERNEL, int C0, int NC, bool EXCL>
-__global__ void __launch_bounds__(TB, 3)
+// Occupancy: the potential (one component) runs 3 wavefronts per SIMD, the
+// potential and gradient (four components, kernel values evaluated once for
+// all four products) 2 (250 VGPRs, no spills).
+template <int KERNEL, int C0, int NC>
+__global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
     k_cc_gemm(const int4 *__restrict__ items, const int *__restrict__ entT, const int *__restrict__ entS,
               LevelBoxes TB_, LevelBoxes SB, const double *__restrict__ W, int n1, NodeTables tabs, KParams kp,
               double *__restrict__ out, size_t comp_stride) {
@@ -467,11 +471,13 @@ __global__ void __launch_bounds__(TB, 3)
     double acc[NC][16];
     float tr[3] = {0.0f, 0.0f, 0.0f};
     int ti = 0;
+    bool idle = false;  // every target of this wavefront is padding (P is not a multiple of 128)
     const float eps2 = (float)kp.eps2;
     for (int st = 0; st < steps; st++) {
         const int t0 = (st % nst) * 64;
         if (t0 == 0) {  // a new block of targets
             ti = (st / nst) * kCcTgt + 32 * w + (lane & 31);
+            idle = (st / nst) * kCcTgt + 32 * w >= P;
             double t[3];
             proxy_point<3>(min(ti, P - 1), n1, tabs, TB_.lo0[T0], TB_.hi0[T0], TB_.lo1[T0], TB_.hi1[T0],
                            TB_.lo2[T0], TB_.hi2[T0], t);
@@ -499,15 +505,26 @@ __global__ void __launch_bounds__(TB, 3)
         f32x16 d[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) d[c] = (f32x16){0};
-#pragma unroll 4
-        for (int step = 0; step < 32; step++) {
-            const int s = 2 * step + par;
+        // The boxes of a CC pair are separated, so no kernel value needs the
+        // zero-distance test, and a padding source (a copy of the tile's last
+        // proxy, finite kernel values) meets a zero weight in A.  A full tile
+        // runs a fixed 32 steps (unrolled), a partial last one only the steps
+        // of its sources.
+        auto step = [&](int k) {
+            const int s = 2 * k + par;
             float g[C];
-            kernel_row<3, KERNEL, EXCL>(tr[0] - sc[0][s], tr[1] - sc[1][s], tr[2] - sc[2][s], eps2, g);
+            kernel_row<3, KERNEL, false>(tr[0] - sc[0][s], tr[1] - sc[1][s], tr[2] - sc[2][s], eps2, g);
             const float q = qt[s][col];
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-                d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, s < nt ? g[C0 + c] : 0.0f, d[c], 0, 0, 0);
+            for (int c = 0; c < NC; c++) d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, g[C0 + c], d[c], 0, 0, 0);
+        };
+        if (!idle) {
+            if (nt == 64) {
+#pragma unroll 4
+                for (int k = 0; k < 32; k++) step(k);
+            } else {
+                for (int k = 0; k < (nt + 1) / 2; k++) step(k);
+            }
         }
 #pragma unroll
         for (int c = 0; c < NC; c++)
@@ -533,14 +550,8 @@ __global__ void __launch_bounds__(TB, 3)
 template <int KERNEL, int C0, int NC>
 void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1,
                     const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
-    if (kp.excl)
-        k_cc_gemm<KERNEL, C0, NC, true><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p,
-                                                                   level_boxes(tt), level_boxes(ts), W, n1, tabs, kp,
-                                                                   Psi, comp_stride);
-    else
-        k_cc_gemm<KERNEL, C0, NC, false><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p,
-                                                                    level_boxes(tt), level_boxes(ts), W, n1, tabs,
-                                                                    kp, Psi, comp_stride);
+    k_cc_gemm<KERNEL, C0, NC><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p, level_boxes(tt),
+                                                        level_boxes(ts), W, n1, tabs, kp, Psi, comp_stride);
     BTK_LAUNCH_CHECK();
 }
 
@@ -550,8 +561,7 @@ void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const doub
     if constexpr (KernelTraits<3, KERNEL>::C == 1) {
         launch_cc_part<KERNEL, 0, 1>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
     } else {
-        launch_cc_part<KERNEL, 0, 2>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
-        launch_cc_part<KERNEL, 2, 2>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_part<KERNEL, 0, 4>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
     }
 }
 

@@ -125,6 +125,9 @@ def parse():
                     help="learned MAC: this per-pair budget instead of the held-out calibration (for profiling)")
     ap.add_argument("--mac-target", type=float, default=0.85e-6,
                     help="learned MAC: rel-L2 the calibration asks of every held-out set (margin below 1e-6)")
+    ap.add_argument("--mac-costs", default="1,1,1,0.35", metavar="PP,PC,CP,CC",
+                    help="learned MAC: relative cost of one kernel evaluation of each pair type (default: a CC "
+                         "evaluation as a shared-matrix MFMA product costs about a third of a vector-ALU one)")
     ap.add_argument("--mac-holdout", type=int, default=3, help="learned MAC: held-out sets for the calibration")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES]",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
@@ -670,6 +673,9 @@ def main():
         ds = direct_sum(btk, torch, dim, kernel, C, d, idx)
 
     res = {}
+    mac_costs = tuple(float(c) for c in args.mac_costs.split(","))
+    if len(mac_costs) != 4:
+        raise SystemExit("--mac-costs takes four values: PP,PC,CP,CC")
     for v in variants:
         cfg = btk.make_config(dim, degree, leaf, theta, 0.0)
         opts = btk.options(kernel=kernel, convention=1, exclude_self=1, precision=prec[v], mac=int(v == "learned"),
@@ -678,6 +684,7 @@ def main():
         train = choice = None
         budget = None
         if v == "learned":  # untimed, like the warmup
+            btk.mac_set_costs(*mac_costs)
             log("learned MAC: samples and training")
             train = train_mac(btk, torch, wl, degree, leaf, theta, args.mac_sets, args.mac_pairs, args.mac_epochs,
                               gen=gen)
@@ -701,7 +708,8 @@ def main():
         res[v] = summarize(stats, ms, n, dtype[v], out, ds, fpe, idx)
         del out
         if v == "learned":
-            res[v].update(mac_budget=budget, mac_budget_choice=choice, mac_training=train)
+            res[v].update(mac_budget=budget, mac_budget_choice=choice, mac_training=train,
+                          mac_costs=dict(zip(("pp", "pc", "cp", "cc"), mac_costs)))
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
                                    "far_tpt": t.far_tpt, "mutual": t.mutual, "near_waves": t.near_waves,
                                    "far_waves": t.far_waves, "autotuned": not (args.no_autotune or args.shapes)}

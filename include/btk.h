@@ -363,6 +363,13 @@ int btk_mac_set_weights(const float *h_weights, int count);
 int btk_mac_get_weights(float *h_weights, int count);
 /* Maximum predicted relative error of an accepted pair (default 1e-7). */
 int btk_mac_set_budget(double rel_err);
+/* Relative cost of one kernel evaluation of a PP, PC, CP and CC pair
+ * (costs[0..3], each > 0; default all 1, so that the learned MAC minimises
+ * evaluations).  The learned MAC takes, among the candidates within the
+ * budget, the one of least weighted cost: with CC products on MFMA
+ * (mixed precision, btk_tuning.cc_group) a CC evaluation costs about a third
+ * of a vector-ALU one.  Process-wide, like the budget. */
+int btk_mac_set_costs(const float *costs);
 /* The installed network on d_X[n][10]: d_out[n][12] (scaled outputs). */
 int btk_mac_forward(const float *d_X, int n, float *d_out, void *stream);
 /* Loss and its gradient (BTK_MAC_WEIGHTS values, the weights' layout) for

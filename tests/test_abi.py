@@ -65,6 +65,15 @@ def test_tuning_round_trip_and_rejects_bad_values():
         btk.restore_tuning(old)
 
 
+def test_mac_costs_are_checked():
+    btk.mac_set_costs(1.0, 1.0, 1.0, 0.35)
+    for bad in ((0.0, 1.0, 1.0, 1.0), (1.0, -1.0, 1.0, 1.0), (1.0, 1.0, float("inf"), 1.0),
+                (1.0, 1.0, 1.0, float("nan"))):
+        with pytest.raises(btk.BtkError, match="positive"):
+            btk.mac_set_costs(*bad)
+    btk.mac_set_costs()
+
+
 def test_components():
     assert btk.components(3, btk.KERNEL_POT) == 1
     assert btk.components(3, btk.KERNEL_REF_VEL3D) == 3

@@ -136,6 +136,54 @@ def test_learned_mixed_precision_vs_oracle_degree_by_degree(cc_group):
     assert rel <= 1e-6
 
 
+def test_cost_weights_steer_the_choice():
+    """btk_mac_set_costs: with a CC evaluation priced at a tenth of the others
+    the learned list takes more CC and fewer PC/CP pairs than with equal
+    prices, and CC pairs with a leaf on one side.  At a hundred times no CC
+    pair has a leaf side: this network predicts one error for every type, so
+    PC or CP (at most THR * P evaluations) is within the budget whenever CC
+    is, and cheaper.  The weighted list's mixed solve (CC as MFMA products)
+    matches the oracle run degree by degree within 1e-6."""
+    print("\n[mac degrees] cost weights", flush=True)
+    x, y, z, q = _problem(9)
+    d = [torch.from_numpy(a).cuda() for a in (x, y, z, q)]
+    cfg = btk.make_config(3, DEG, THR, THETA, 0.0)
+    tree = btk.Tree(cfg, d[0], d[1], d[2])
+    counts = {}
+    pairs = {}
+    try:
+        for cc in (1.0, 0.1, 100.0):
+            btk.mac_set_costs(1.0, 1.0, 1.0, cc)
+            p = btk.InteractionList(cfg, tree, tree, learned_mac=True).pairs()
+            ty = p["interact_type"] & 3
+            counts[cc] = [int(np.sum(ty == k)) for k in range(4)]
+            pairs[cc] = p
+            print(f"  CC cost {cc}: PP/PC/CP/CC pairs {counts[cc]}", flush=True)
+        assert counts[0.1][3] > counts[1.0][3]
+        assert counts[0.1][1] + counts[0.1][2] < counts[1.0][1] + counts[1.0][2]
+        cnt = tree.panels()["point_count"]
+
+        def leaf_side_cc(p):
+            cc = (p["interact_type"] & 3) == 3
+            return int(np.sum(cc & ((cnt[p["target_panel"]] <= THR) | (cnt[p["source_panel"]] <= THR))))
+
+        print(f"  CC pairs with a leaf side: cost 0.1 {leaf_side_cc(pairs[0.1])}, "
+              f"cost 100 {leaf_side_cc(pairs[100.0])}", flush=True)
+        assert leaf_side_cc(pairs[0.1]) > 0 and leaf_side_cc(pairs[100.0]) == 0
+        btk.mac_set_costs(1.0, 1.0, 1.0, 0.1)
+        C = btk.components(3, btk.KERNEL_POT_GRAD)
+        out = torch.zeros(C * N, dtype=torch.float64, device="cuda")
+        st = btk.solve(cfg, btk.options(kernel=btk.KERNEL_POT_GRAD, exclude_self=1, mac=1, precision=btk.PREC_MIXED),
+                       d[0], d[1], d[2], d[3], out)
+        assert [int(st.count[k]) for k in range(4)] == counts[0.1]
+    finally:
+        btk.mac_set_costs()
+    ref = _oracle_by_degree(pairs[0.1], btk.KERNEL_POT_GRAD, x, y, z, q)
+    rel = np.linalg.norm(out.view(C, N).cpu().numpy() - ref) / np.linalg.norm(ref)
+    print(f"  CC pairs as shared-matrix products {int(st.cc_shared)}; rel vs oracle {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+
+
 def test_learned_plan_and_batch_match_the_solve():
     print("\n[mac degrees] plan and batch vs solve", flush=True)
     x, y, z, q = _problem(6)
