@@ -393,6 +393,27 @@ __global__ void k_cc_fill(int G, const int *__restrict__ gstart, const int *__re
         items[ibase[g] + it] = make_int4(gstart[g], size, 0, kCcCols * it);
 }
 
+// Work items in the order of the target cluster of their first pair (stable:
+// keys in key order within one cluster).  A key holds each target cluster at
+// most once, so a cluster's proxy sums receive one item of each of its keys;
+// in key order those items lie a whole level's clusters apart, in this order
+// they run close together and the sums' cache lines are still in L2 / MALL
+// when the next item adds into them.
+__global__ void k_cc_item_keys(int n, const int4 *__restrict__ items, const int *__restrict__ entT,
+                               int *__restrict__ key, int *__restrict__ val) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const int4 it = items[i];
+    key[i] = entT[it.x + it.w];
+    val[i] = i;
+}
+
+__global__ void k_cc_item_gather(int n, const int *__restrict__ order, const int4 *__restrict__ in,
+                                 int4 *__restrict__ out) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i < n) out[i] = in[order[i]];
+}
+
 // Panels of every keyed entry in key order; the entries of the selected keys
 // leave the regular far kernel (source count zero in meta).
 __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__restrict__ goff_g, int pct,
@@ -428,7 +449,8 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
     constexpr int C = KernelTraits<3, KERNEL>::C;
     constexpr int QPT = 64 * kCcCols / TB;  // weights per thread per tile
     static_assert(C0 + NC <= C, "component range");
-    __shared__ float sc[3][64];
+    constexpr int kMaxP = kMaxN1 * kMaxN1 * kMaxN1;
+    __shared__ float sc[3][kMaxP];  // every source proxy of the key's first pair, relative to ctr
     __shared__ float qt[64][kCcCols + 1];
     __shared__ double cn[3][kMaxN1];
     __shared__ int sS[kCcCols], sT[kCcCols];
@@ -451,14 +473,26 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
                               sel3(d, SB.hi0[S0], SB.hi1[S0], SB.hi2[S0]));
     }
     __syncthreads();
-    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64
+    // source proxy coordinates relative to the target box centre, once per
+    // item (no integer division by n1 in the tile loop)
+    for (int j = tid; j < P; j += TB) {
+        sc[0][j] = (float)(cn[0][j / (n1 * n1)] - ctr[0]);
+        sc[1][j] = (float)(cn[1][(j / n1) % n1] - ctr[1]);
+        sc[2][j] = (float)(cn[2][j % n1] - ctr[2]);
+    }
+    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64;
+    // every load is issued (padding reads the key's first weight and is
+    // dropped), so the loads of a tile overlap and nothing waits for them
+    // before the tile is stored
     auto load_q = [&](int t0, float *q) {
 #pragma unroll
         for (int i = 0; i < QPT; i++) {
             const int k = tid + TB * i;
             const int s = k % 64, r = k / 64;
             const int S = sS[r];
-            q[i] = (t0 + s < P && S >= 0) ? (float)W[(size_t)S * P + t0 + s] : 0.0f;
+            const bool ok = t0 + s < P && S >= 0;
+            const double v = W[(size_t)(ok ? S : S0) * P + (ok ? t0 + s : 0)];
+            q[i] = ok ? (float)v : 0.0f;
         }
     };
     // the item runs every block of kCcTgt targets against every 64-source
@@ -489,12 +523,6 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
                 for (int v = 0; v < 16; v++) acc[c][v] = 0.0;
         }
         const int nt = min(64, P - t0);
-        if (tid < 64) {
-            const int j = min(t0 + tid, P - 1);
-            sc[0][tid] = (float)(cn[0][j / (n1 * n1)] - ctr[0]);
-            sc[1][tid] = (float)(cn[1][(j / n1) % n1] - ctr[1]);
-            sc[2][tid] = (float)(cn[2][j % n1] - ctr[2]);
-        }
 #pragma unroll
         for (int i = 0; i < QPT; i++) {
             const int k = tid + TB * i;
@@ -503,33 +531,37 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
         __syncthreads();
         if (st + 1 < steps) load_q(((st + 1) % nst) * 64, qn);
         f32x16 d[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) d[c] = (f32x16){0};
         // The boxes of a CC pair are separated, so no kernel value needs the
-        // zero-distance test, and a padding source (a copy of the tile's last
+        // zero-distance test, and a padding source (a copy of the item's last
         // proxy, finite kernel values) meets a zero weight in A.  A full tile
         // runs a fixed 32 steps (unrolled), a partial last one only the steps
-        // of its sources.
-        auto step = [&](int k) {
+        // of its sources.  The first step of a tile starts the FP32 sums
+        // from a zero operand.
+        auto step = [&](int k, auto first, bool clamp) {
             const int s = 2 * k + par;
+            const int j = clamp ? min(t0 + s, P - 1) : t0 + s;
             float g[C];
-            kernel_row<3, KERNEL, false>(tr[0] - sc[0][s], tr[1] - sc[1][s], tr[2] - sc[2][s], eps2, g);
+            kernel_row<3, KERNEL, false>(tr[0] - sc[0][j], tr[1] - sc[1][j], tr[2] - sc[2][j], eps2, g);
             const float q = qt[s][col];
 #pragma unroll
-            for (int c = 0; c < NC; c++) d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, g[C0 + c], d[c], 0, 0, 0);
+            for (int c = 0; c < NC; c++)
+                d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, g[C0 + c], decltype(first)::value ? (f32x16){0} : d[c],
+                                                            0, 0, 0);
         };
         if (!idle) {
             if (nt == 64) {
+                step(0, std::true_type{}, false);
 #pragma unroll 4
-                for (int k = 0; k < 32; k++) step(k);
+                for (int k = 1; k < 32; k++) step(k, std::false_type{}, false);
             } else {
-                for (int k = 0; k < (nt + 1) / 2; k++) step(k);
+                step(0, std::true_type{}, true);
+                for (int k = 1; k < (nt + 1) / 2; k++) step(k, std::false_type{}, true);
             }
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[c][v] += (double)d[c][v];
         }
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) acc[c][v] += (double)d[c][v];
         __syncthreads();
         // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
         if (t0 + 64 >= P && ti < P) {  // the block's last source tile: its sums are complete
@@ -765,6 +797,18 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int mi
     BTK_LAUNCH_CHECK();
     k_cc_entries<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, gid.p, nitems.p, meta, cg.ent_T.p, cg.ent_S.p);
     BTK_LAUNCH_CHECK();
+    {
+        DBuf<int> ik(total), ik2(total), iv(total), iv2(total);
+        DBuf<int4> unsorted(total);
+        k_cc_item_keys<<<nblk(total), TB, 0, s>>>(total, cg.items.p, cg.ent_T.p, ik.p, iv.p);
+        BTK_LAUNCH_CHECK();
+        int tbits = 1;
+        while ((1ll << tbits) < pct) tbits++;
+        sort_pairs_i32(ik.p, ik2.p, iv.p, iv2.p, total, tbits, s);
+        BTK_HIP(hipMemcpyAsync(unsorted.p, cg.items.p, sizeof(int4) * total, hipMemcpyDeviceToDevice, s));
+        k_cc_item_gather<<<nblk(total), TB, 0, s>>>(total, iv2.p, unsorted.p, cg.items.p);
+        BTK_LAUNCH_CHECK();
+    }
     cg.n_items = total;
 }
 
