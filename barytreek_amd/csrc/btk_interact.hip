@@ -218,6 +218,7 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             BTK_HIP(hipEventRecord(kev->ev[2], s));
         }
     }
+    if (p.w32.n) weights_to_f32(p.w32.n, W, p.w32.p, s);  // the CC products' A operand
     for (int far = 0; far < 2; far++) {
         bool any = false;
         for (int di = 0; di < p.ndeg; di++) any |= p.g[far][di].present;
@@ -253,7 +254,8 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
             if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, p.sorted_out, s);
             if (g.cc.n_items)
-                launch_ccgemm(g.cc, tt, ts, W + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride, s);
+                launch_ccgemm(g.cc, tt, ts, p.w32.p + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride,
+                              s);
             if (g.tk.n && !queued) launch_interact<DIM, KERNEL>(L, s);
         }
         if (kev && !queued) {
@@ -516,6 +518,9 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             g.present = g.tk.n > 0 || g.mutual || g.cc.n_items > 0;
         }
     }
+    bool any_cc = false;
+    for (int di = 0; di < l.ndeg; di++) any_cc |= p->g[1][di].cc.n_items > 0;
+    if (any_cc) p->w32.resize(proxy_offset(l, ts.pc, 1, l.ndeg));
     if (o.schedule == BTK_SCHEDULE_QUEUE) build_queue(*p, l, tt, s);
     return p;
 }

@@ -554,8 +554,11 @@ struct CCGemm {
 constexpr int kCcMinGroup = 8;
 void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,
                   CCGemm &cg, hipStream_t s);
-void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1, const NodeTables &tabs,
+// W32: the source proxy weights of the group's degree rounded to FP32 (the
+// products' A operand), as written by weights_to_f32.
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W32, int n1, const NodeTables &tabs,
                    const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s);
+void weights_to_f32(size_t n, const double *W, float *W32, hipStream_t s);
 
 // Everything the interaction launches of one list need besides the charges,
 // the weights and the outputs: entry descriptors and the work items of the
@@ -582,6 +585,7 @@ struct InteractPlan {
     } g[2][4];  // [far][degree index]
     long long cc_shared = 0;  // CC entries evaluated through shared kernel matrices
     bool sorted_out = false;  // near output in the target tree's sorted order
+    DBuf<float> w32;          // FP32 copy of every degree's source weights, when a group has CC products
     // btk_options.schedule = BTK_SCHEDULE_QUEUE: the work items of every
     // group as one queue, biggest first, drained by one persistent launch
     struct Queue {

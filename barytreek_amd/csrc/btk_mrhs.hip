@@ -444,7 +444,7 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
 template <int KERNEL, int C0, int NC>
 __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
     k_cc_gemm(const int4 *__restrict__ items, const int *__restrict__ entT, const int *__restrict__ entS,
-              LevelBoxes TB_, LevelBoxes SB, const double *__restrict__ W, int n1, NodeTables tabs, KParams kp,
+              LevelBoxes TB_, LevelBoxes SB, const float *__restrict__ W, int n1, NodeTables tabs, KParams kp,
               double *__restrict__ out, size_t comp_stride) {
     constexpr int C = KernelTraits<3, KERNEL>::C;
     constexpr int QPT = 64 * kCcCols / TB;  // weights per thread per tile
@@ -480,19 +480,20 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
         sc[1][j] = (float)(cn[1][(j / n1) % n1] - ctr[1]);
         sc[2][j] = (float)(cn[2][j % n1] - ctr[2]);
     }
-    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64;
-    // every load is issued (padding reads the key's first weight and is
-    // dropped), so the loads of a tile overlap and nothing waits for them
-    // before the tile is stored
+    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64.
+    // Every load is issued from an address in bounds and used as loaded, so
+    // the loads overlap the tile's products and are waited for only when the
+    // next tile is stored: a padding pair row reads the key's first weights
+    // (its output row is never written; rows of a product do not mix), a
+    // padding source the item's last proxy weight (its kernel values are
+    // zero, see step)
     auto load_q = [&](int t0, float *q) {
 #pragma unroll
         for (int i = 0; i < QPT; i++) {
             const int k = tid + TB * i;
             const int s = k % 64, r = k / 64;
             const int S = sS[r];
-            const bool ok = t0 + s < P && S >= 0;
-            const double v = W[(size_t)(ok ? S : S0) * P + (ok ? t0 + s : 0)];
-            q[i] = ok ? (float)v : 0.0f;
+            q[i] = W[(size_t)(S >= 0 ? S : S0) * P + min(t0 + s, P - 1)];
         }
     };
     // the item runs every block of kCcTgt targets against every 64-source
@@ -532,16 +533,19 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
         if (st + 1 < steps) load_q(((st + 1) % nst) * 64, qn);
         f32x16 d[NC];
         // The boxes of a CC pair are separated, so no kernel value needs the
-        // zero-distance test, and a padding source (a copy of the item's last
-        // proxy, finite kernel values) meets a zero weight in A.  A full tile
-        // runs a fixed 32 steps (unrolled), a partial last one only the steps
-        // of its sources.  The first step of a tile starts the FP32 sums
-        // from a zero operand.
+        // zero-distance test.  A full tile runs a fixed 32 steps (unrolled),
+        // a partial last one only the steps of its sources, where a padding
+        // source (at the item's last proxy) has zero kernel values.  The
+        // first step of a tile starts the FP32 sums from a zero operand.
         auto step = [&](int k, auto first, bool clamp) {
             const int s = 2 * k + par;
             const int j = clamp ? min(t0 + s, P - 1) : t0 + s;
             float g[C];
             kernel_row<3, KERNEL, false>(tr[0] - sc[0][j], tr[1] - sc[1][j], tr[2] - sc[2][j], eps2, g);
+            if (clamp && t0 + s >= P) {
+#pragma unroll
+                for (int c = 0; c < C; c++) g[c] = 0.0f;
+            }
             const float q = qt[s][col];
 #pragma unroll
             for (int c = 0; c < NC; c++)
@@ -580,7 +584,7 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
 }
 
 template <int KERNEL, int C0, int NC>
-void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1,
+void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1,
                     const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
     k_cc_gemm<KERNEL, C0, NC><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p, level_boxes(tt),
                                                         level_boxes(ts), W, n1, tabs, kp, Psi, comp_stride);
@@ -588,7 +592,7 @@ void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const doub
 }
 
 template <int KERNEL>
-void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1,
+void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1,
                     const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
     if constexpr (KernelTraits<3, KERNEL>::C == 1) {
         launch_cc_part<KERNEL, 0, 1>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
@@ -812,7 +816,18 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int mi
     cg.n_items = total;
 }
 
-void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const double *W, int n1, const NodeTables &tabs,
+__global__ void k_weights_f32(size_t n, const double *__restrict__ W, float *__restrict__ W32) {
+    for (size_t i = (size_t)blockIdx.x * TB + threadIdx.x; i < n; i += (size_t)gridDim.x * TB) W32[i] = (float)W[i];
+}
+
+void weights_to_f32(size_t n, const double *W, float *W32, hipStream_t s) {
+    if (n == 0) return;
+    const int blocks = (int)std::min<size_t>((n + TB - 1) / TB, 8192);
+    k_weights_f32<<<blocks, TB, 0, s>>>(n, W, W32);
+    BTK_LAUNCH_CHECK();
+}
+
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1, const NodeTables &tabs,
                    const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s) {
     if (cg.n_items == 0) return;
     if (kernel == BTK_KERNEL_POT)
