@@ -376,9 +376,15 @@ struct DegreeMatrix {
 
 __global__ void __launch_bounds__(TB)
     k_degree_change(int na, int nb, DegreeMatrix A, const double *__restrict__ in, double *__restrict__ out,
-                    int add) {
+                    int add, const int *__restrict__ goff, int pct) {
     __shared__ double X[kMaxP], Y[kMaxP];
     const size_t b = blockIdx.x;
+    // with goff (the panels' entries of one group): a panel without entries
+    // holds zeros in `in` and adds nothing
+    if (goff) {
+        const int p = (int)(b % pct);
+        if (goff[p + 1] == goff[p]) return;
+    }
     const int a2 = na * na, Pa = a2 * na, Pb = nb * nb * nb;
     for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[b * Pa + i];
     __syncthreads();
@@ -515,7 +521,7 @@ void upward_all(const btk_run_config &cfg, const List &l, const Tree &t, const d
         // W_d[k] = sum_m W_n[m] L_k(y_m): A[k][m], degree d out, degree n in
         const int d = l.deg[di];
         const DegreeMatrix A = transposed(degree_matrix(d, n), d, n);
-        k_degree_change<<<t.pc, TB, 0, s>>>(n + 1, d + 1, A, Wn, W + proxy_offset(l, t.pc, 1, di), 0);
+        k_degree_change<<<t.pc, TB, 0, s>>>(n + 1, d + 1, A, Wn, W + proxy_offset(l, t.pc, 1, di), 0, nullptr, 0);
         BTK_LAUNCH_CHECK();
     }
 }
@@ -529,10 +535,14 @@ void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l
     if (top > 0 && t.pc > 0) {
         BTK_REQUIRE(t.dim == 3, "several interpolation degrees: 3D only");
         for (int di = 0; di < top; di++) {
-            // Psi_n[m] += sum_k L_k(y_m) Psi_d[k]: degree n out, degree d in
+            // Psi_n[m] += sum_k L_k(y_m) Psi_d[k]: degree n out, degree d in.
+            // Only the far (CP, CC) entries of degree d write Psi_d, so a
+            // panel without such entries is skipped.
             const int d = l.deg[di];
+            BTK_REQUIRE(l.pct == t.pc, "the list's target panels are not this tree's");
             k_degree_change<<<C * t.pc, TB, 0, s>>>(d + 1, n + 1, degree_matrix(d, n),
-                                                    Psi + proxy_offset(l, t.pc, C, di), Pn, 1);
+                                                    Psi + proxy_offset(l, t.pc, C, di), Pn, 1,
+                                                    l.goff.p + (size_t)(l.ndeg + di) * t.pc, t.pc);
             BTK_LAUNCH_CHECK();
         }
     }
