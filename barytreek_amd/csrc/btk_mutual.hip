@@ -150,8 +150,11 @@ constexpr int kMaxWaves = 8;
 // and writes them (one atomic add per point and component).
 constexpr int kTiles = 4;
 
-template <int DIM, int KERNEL, int TPT>
-__global__ void __launch_bounds__(kLanes * kMaxWaves)
+// WMAX: the largest wavefront count the launch may use (4 or 8); it sizes the
+// cross-wavefront reduction buffer, so a launch of 4 wavefronts per workgroup
+// takes half the LDS and runs more workgroups per CU.
+template <int DIM, int KERNEL, int TPT, int WMAX>
+__global__ void __launch_bounds__(kLanes * WMAX)
     k_mutual(TaskView tk, EntryView mv, const int *__restrict__ start, const int *__restrict__ count,
              const int *__restrict__ perm, const double *__restrict__ px, const double *__restrict__ py,
              const double *__restrict__ pz, const double *__restrict__ q, KParams kp, double *__restrict__ out,
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(kLanes * kMaxWaves)
     constexpr int SLOTS = kTiles * 2 * kLanes;
     __shared__ double lx[SLOTS], ly[SLOTS], lz[DIM == 3 ? SLOTS : 1];
     __shared__ float lq[SLOTS];
-    __shared__ float red[C][kMaxWaves][kTiles * kLanes];
+    __shared__ float red[C][WMAX][kTiles * kLanes];
     const int tid = threadIdx.x;
     const int lane = tid % kLanes;
     const int w = tid / kLanes;
@@ -296,14 +299,22 @@ void launch_mutual(const Mutual &mu, const Tree &t, const double *q_sorted, cons
     const TaskView tk{mu.tasks.panel.p, mu.tasks.first.p, mu.tasks.e0.p, mu.tasks.e1.p};
     const EntryView mv{mu.mmeta.p};
     const int *perm = sorted_out ? nullptr : t.perm.p;
-#define BTK_MU(TPT_)                                                                                          \
-    k_mutual<DIM, KERNEL, TPT_><<<mu.n, kLanes * mu.waves, 0, s>>>(tk, mv, t.start.p, t.count.p, perm,       \
-                                                                    t.sx.p, t.sy.p, t.sz.p, q_sorted, kp, out, \
-                                                                    stride)
-    if (mu.tpt == 2)
-        BTK_MU(2);
-    else
-        BTK_MU(1);
+#define BTK_MU(TPT_, WMAX_)                                                                                   \
+    k_mutual<DIM, KERNEL, TPT_, WMAX_><<<mu.n, kLanes * mu.waves, 0, s>>>(tk, mv, t.start.p, t.count.p, perm, \
+                                                                           t.sx.p, t.sy.p, t.sz.p, q_sorted,   \
+                                                                           kp, out, stride)
+    BTK_REQUIRE(mu.waves >= 1 && mu.waves <= kMaxWaves, "mutual pairs: 1 to 8 wavefronts per workgroup");
+    if (mu.tpt == 2) {
+        if (mu.waves <= 4)
+            BTK_MU(2, 4);
+        else
+            BTK_MU(2, 8);
+    } else {
+        if (mu.waves <= 4)
+            BTK_MU(1, 4);
+        else
+            BTK_MU(1, 8);
+    }
 #undef BTK_MU
     BTK_LAUNCH_CHECK();
 }
