@@ -90,16 +90,23 @@ def test_results_independent_of_launch_shapes(dim):
         assert float(torch.linalg.norm(out - fp64[0]) / torch.linalg.norm(fp64[0])) < 1e-13
 
 
-@pytest.mark.parametrize("dim,kernel", [(2, btk.KERNEL_POT), (2, btk.KERNEL_POT_GRAD), (3, btk.KERNEL_POT),
-                                        (3, btk.KERNEL_REF_VEL3D), (3, btk.KERNEL_POT_GRAD)])
-def test_mutual_pairs_match_one_way_evaluation(dim, kernel):
+@pytest.mark.parametrize("dim,kernel,leaf", [(2, btk.KERNEL_POT, 250), (2, btk.KERNEL_POT_GRAD, 250),
+                                             (3, btk.KERNEL_POT, 250), (3, btk.KERNEL_REF_VEL3D, 250),
+                                             (3, btk.KERNEL_POT_GRAD, 250), (3, btk.KERNEL_POT_GRAD, 2000)])
+def test_mutual_pairs_match_one_way_evaluation(dim, kernel, leaf):
     """Mixed precision: evaluating mutual PP pairs once for both directions
     gives the one-way (per entry) result to FP32 rounding, for every kernel
-    and both lane shapes; both are within 1e-6 of FP64."""
-    print(f"\n[tuning] mutual dim={dim} kernel={kernel}", flush=True)
-    n = 40000
-    d = _problem(dim, n, 5)
-    cfg = btk.make_config(dim, 5, 250, 0.7, 0.0)
+    and both lane shapes; both are within 1e-6 of FP64.  The leaf-2000 case
+    (64 uniform-cube leaves of about 1900 points) takes 8 wavefronts per
+    workgroup, the largest reduction buffer; the leaf-250 cases at most 4."""
+    print(f"\n[tuning] mutual dim={dim} kernel={kernel} leaf={leaf}", flush=True)
+    if leaf <= 250:
+        n = 40000
+        d = _problem(dim, n, 5)
+    else:
+        n = 120000
+        d = [torch.from_numpy(a).cuda() for a in wl.uniform_cube(n, 3, 5)]
+    cfg = btk.make_config(dim, 5, leaf, 0.7, 0.0)
     C = btk.components(dim, kernel)
 
     def run(prec, **tu):
