@@ -115,6 +115,45 @@ __device__ __forceinline__ double log_f64(double x) {
     return x > 0.0 ? r : -__builtin_inf();  // log 0 = -inf, as the C library's
 }
 
+// Table-driven FP64 log (Tang, "Table-driven implementation of the logarithm
+// function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990) for the
+// 2D FP64 kernels, where log is most of an evaluation.  x = 2^e m with m in
+// [1, 2); entry j of kLogTab covers m in [1 + j/128, 1 + (j+1)/128) and holds
+// inv = 1/c (c the interval's midpoint) rounded to 9 bits and L = -log(inv).
+// r = m inv - 1 is formed by one fma (m inv is exact in its 63 bits), |r| <
+// 0.006, and log x = e ln 2 + L + log1p(r) with log1p by its Taylor series to
+// r^8 (remainder below 2^-60 relative).  22 instructions against the
+// reduction above's 34; absolute error a few units of 2^-53 |log x| or of
+// 2^-53 ln 2 (near x = 1, where the terms cancel).  Each workgroup fills
+// its table with log_f64 (fill_log_tab) before use.
+constexpr int kLogTab = 128;
+
+__device__ __forceinline__ void fill_log_tab(double2 *tab, int tid, int nthreads) {
+    for (int j = tid; j < kLogTab; j += nthreads) {
+        const double c = 1.0 + (j + 0.5) / kLogTab;
+        const double inv = rint(512.0 / c) / 512.0;
+        tab[j] = make_double2(-log_f64(inv), inv);
+    }
+}
+
+__device__ __forceinline__ double log_f64_tab(double x, const double2 *tab) {
+    const double m = 2.0 * __builtin_amdgcn_frexp_mant(x);  // [1, 2)
+    const int e = __builtin_amdgcn_frexp_exp(x) - 1;
+    const int j = (__double2hiint(m) >> 13) & (kLogTab - 1);  // the top 7 bits of m's fraction
+    const double2 t = tab[j];
+    const double r = fma(m, t.y, -1.0);
+    double p = fma(r, -1.0 / 8.0, 1.0 / 7.0);
+    p = fma(p, r, -1.0 / 6.0);
+    p = fma(p, r, 1.0 / 5.0);
+    p = fma(p, r, -1.0 / 4.0);
+    p = fma(p, r, 1.0 / 3.0);
+    p = fma(p, r, -0.5);
+    const double lp = fma(r * r, p, r);  // log1p(r)
+    const double de = (double)e;
+    const double res = fma(de, kLn2Hi, fma(de, kLn2Lo, t.x + lp));
+    return x > 0.0 ? res : -__builtin_inf();
+}
+
 // ---------------------------------------------------------------------------
 // Kernel evaluations.  acc[] collects the sums without the constant factor;
 // finish() applies it.  Components:
@@ -130,8 +169,10 @@ struct KernelTraits {
 
 // d = t - s is passed in; EXCL skips the pair when R == eps^2, i.e. when
 // |d|^2 vanishes at working precision (the self pair of coincident sets).
-template <int DIM, int KERNEL, bool EXCL>
-__device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double q, double eps2, double *acc) {
+// LOGTAB: 2D log by the table ltab (log_f64_tab) instead of log_f64.
+template <int DIM, int KERNEL, bool EXCL, bool LOGTAB = false>
+__device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double q, double eps2, double *acc,
+                                         const double2 *ltab = nullptr) {
     if constexpr (DIM == 3) {
         const double R = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
         const bool skip = EXCL && R == eps2;
@@ -156,7 +197,11 @@ __device__ __forceinline__ void eval_f64(double dx, double dy, double dz, double
         (void)dz;
         const double R = fma(dx, dx, fma(dy, dy, eps2));
         const bool skip = EXCL && R == eps2;
-        const double lg = skip ? 0.0 : log_f64(R);
+        double lg = 0.0;
+        if constexpr (LOGTAB)
+            lg = skip ? 0.0 : log_f64_tab(R, ltab);
+        else
+            lg = skip ? 0.0 : log_f64(R);
         acc[0] = fma(q, lg, acc[0]);
         if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
             const double g = skip ? 0.0 : q * recip_f64(R);

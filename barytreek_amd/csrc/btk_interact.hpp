@@ -158,9 +158,10 @@ __device__ __forceinline__ void proxy_point(int idx, int n1, const NodeTables &t
 
 // The FP64 path: t - s and the kernel in FP64.
 template <int DIM, int KERNEL, bool EXCL>
-__device__ __forceinline__ void eval_pair(const TileSrc<false> &s, const double *t, double eps2, double *acc64) {
+__device__ __forceinline__ void eval_pair(const TileSrc<false> &s, const double *t, double eps2, double *acc64,
+                                          const double2 *ltab) {
     const double dx = t[0] - s.x, dy = t[1] - s.y, dz = DIM == 3 ? t[2] - s.z : 0.0;
-    eval_f64<DIM, KERNEL, EXCL>(dx, dy, dz, s.q, eps2, acc64);
+    eval_f64<DIM, KERNEL, EXCL, DIM == 2>(dx, dy, dz, s.q, eps2, acc64, ltab);
 }
 
 // Source j of entry m: a point of a source leaf, or a proxy point of a
@@ -229,7 +230,8 @@ template <int DIM, int KERNEL, bool F32, int TPT, bool FAR, bool EXCL, int WAVES
 __device__ __forceinline__ void interact_item(int item, const TaskView &tk, const EntryView &ev, const TgtView &tv,
                                               const SrcView &sv, int n1, const NodeTables &tabs, const KParams &kp,
                                               double *__restrict__ out, size_t comp_stride, int atomic,
-                                              int same_tree, TileOf<DIM, F32, FAR> *tile) {
+                                              int same_tree, TileOf<DIM, F32, FAR> *tile,
+                                              const double2 *ltab = nullptr) {
     constexpr int C = KernelTraits<DIM, KERNEL>::C;
     constexpr bool REL = F32 && FAR;                 // relative FP32 coordinates (TileRel)
     constexpr bool MIX = F32 && !FAR;                // FP64 or relative FP32 coordinates (TileMix)
@@ -325,7 +327,7 @@ __device__ __forceinline__ void interact_item(int item, const TaskView &tk, cons
                 f32((float)(t[k][0] - s.x), (float)(t[k][1] - s.y), 0.0f);
         } else {
             static_assert(!F32, "mixed precision takes TileRel (far) or TileMix (near)");
-            eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k]);
+            eval_pair<DIM, KERNEL, X>(s, t[k], kp.eps2, acc[k], ltab);
         }
     };
     // The entries' sources stream through the tile TS at a time, as one
@@ -443,9 +445,15 @@ __global__ void __launch_bounds__(kLanes * WAVES, F32 ? 6 : 0)
     k_interact(TaskView tk, EntryView ev, TgtView tv, SrcView sv, int n1, NodeTables tabs, KParams kp,
                double *__restrict__ out, size_t comp_stride, int atomic, int same_tree) {
     __shared__ TileOf<DIM, F32, FAR> tile[kLanes * WAVES];
+    constexpr bool LOGTAB = DIM == 2 && !F32;  // FP64 2D: the table-driven log
+    __shared__ double2 ltab[LOGTAB ? kLogTab : 1];
+    if constexpr (LOGTAB) {
+        fill_log_tab(ltab, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
     interact_item<DIM, KERNEL, F32, TPT, FAR, EXCL, WAVES, KAHAN>(xcd_contiguous(blockIdx.x, gridDim.x), tk, ev, tv,
                                                                   sv, n1, tabs, kp, out, comp_stride, atomic,
-                                                                  same_tree, tile);
+                                                                  same_tree, tile, LOGTAB ? ltab : nullptr);
 }
 
 // One group (near or far, one degree) of a work queue.  Its source proxy
@@ -469,6 +477,9 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
     __shared__ TileOf<DIM, F32, false> tn[kLanes];
     __shared__ TileOf<DIM, F32, true> tf[kLanes];
     __shared__ int s_item;
+    constexpr bool LOGTAB = DIM == 2 && !F32;  // FP64 2D: the table-driven log
+    __shared__ double2 ltab[LOGTAB ? kLogTab : 1];
+    if constexpr (LOGTAB) fill_log_tab(ltab, threadIdx.x, blockDim.x);  // the loop's first barrier orders it
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1);
         __syncthreads();
@@ -481,11 +492,12 @@ __global__ void __launch_bounds__(kLanes, F32 ? 6 : 0)
         s2.W = W + g.woff;
         if (g.far)
             interact_item<DIM, KERNEL, F32, FTPT, true, EXCL, 1>(gi.y, g.tk, ev, tv, s2, g.n1, g.tabs, kp, Psi + g.ooff,
-                                                                 g.comp_stride, g.atomic, 0, tf);
+                                                                 g.comp_stride, g.atomic, 0, tf,
+                                                                 LOGTAB ? ltab : nullptr);
         else
             interact_item<DIM, KERNEL, F32, NTPT, false, EXCL, 1>(gi.y, g.tk, ev, tv, s2, g.n1, g.tabs, kp,
                                                                   out + g.ooff, g.comp_stride, g.atomic, same_tree,
-                                                                  tn);
+                                                                  tn, LOGTAB ? ltab : nullptr);
     }
 }
 
