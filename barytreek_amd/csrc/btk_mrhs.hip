@@ -435,9 +435,12 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
 // registers), the B operand the kernel of the key's first pair (FP32
 // coordinates relative to its target box centre, as the far kernel's
 // TileRel) -- lane l evaluates it for target l%32 and source parity l/32.
-// FP32 tile products are added into FP64 sums; each accumulator row is one
-// pair, its lanes 32 consecutive target proxies, so the FP64 atomic adds
-// into each pair's own target cluster are coalesced.
+// Summation is blocked in two FP32 levels: the matrix cores sum one 64-source
+// tile, the tile sums are added in FP32 registers (a pair's sum over P
+// proxies carries about 64 + P/64 roundings instead of P), and each pair's
+// sum is promoted to FP64 once, in the atomic add into its target cluster.
+// Each accumulator row is one pair, its lanes 32 consecutive target
+// proxies, so those FP64 atomic adds are coalesced.
 // Occupancy: the potential (one component) runs 3 wavefronts per SIMD, the
 // potential and gradient (four components, kernel values evaluated once for
 // all four products) 2 (250 VGPRs, no spills).
@@ -503,7 +506,7 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
     const int steps = nst * ((P + kCcTgt - 1) / kCcTgt);
     float qn[QPT];
     load_q(0, qn);
-    double acc[NC][16];
+    float acc[NC][16];
     float tr[3] = {0.0f, 0.0f, 0.0f};
     int ti = 0;
     bool idle = false;  // every target of this wavefront is padding (P is not a multiple of 128)
@@ -521,7 +524,7 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
 #pragma unroll
             for (int c = 0; c < NC; c++)
 #pragma unroll
-                for (int v = 0; v < 16; v++) acc[c][v] = 0.0;
+                for (int v = 0; v < 16; v++) acc[c][v] = 0.0f;
         }
         const int nt = min(64, P - t0);
 #pragma unroll
@@ -564,7 +567,7 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
 #pragma unroll
             for (int c = 0; c < NC; c++)
 #pragma unroll
-                for (int v = 0; v < 16; v++) acc[c][v] += (double)d[c][v];
+                for (int v = 0; v < 16; v++) acc[c][v] += d[c][v];
         }
         __syncthreads();
         // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
@@ -576,7 +579,7 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
                     const double scale = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc);
-                    unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * acc[c][v]);
+                    unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * (double)acc[c][v]);
                 }
             }
         }
