@@ -368,23 +368,16 @@ void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const Nod
 // Change of interpolation degree, one workgroup per proxy array (a panel's
 // charges, or one component of a panel's potentials): out = (A x A x A) in
 // with A the nb x na 1-D matrix, applied as three 1-D contractions through
-// LDS; `add` accumulates into out.  Layout (j * n + k) * n + l as the passes.
+// LDS.  Layout (j * n + k) * n + l as the passes.
 // ---------------------------------------------------------------------------
 struct DegreeMatrix {
     double a[kMaxN1 * kMaxN1];  // row-major nb x na
 };
 
 __global__ void __launch_bounds__(TB)
-    k_degree_change(int na, int nb, DegreeMatrix A, const double *__restrict__ in, double *__restrict__ out,
-                    int add, const int *__restrict__ goff, int pct) {
+    k_degree_change(int na, int nb, DegreeMatrix A, const double *__restrict__ in, double *__restrict__ out) {
     __shared__ double X[kMaxP], Y[kMaxP];
     const size_t b = blockIdx.x;
-    // with goff (the panels' entries of one group): a panel without entries
-    // holds zeros in `in` and adds nothing
-    if (goff) {
-        const int p = (int)(b % pct);
-        if (goff[p + 1] == goff[p]) return;
-    }
     const int a2 = na * na, Pa = a2 * na, Pb = nb * nb * nb;
     for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[b * Pa + i];
     __syncthreads();
@@ -406,10 +399,73 @@ __global__ void __launch_bounds__(TB)
         const int jk = i / nb, l2 = i % nb;
         double v = 0.0;
         for (int l1 = 0; l1 < na; l1++) v = fma(A.a[l2 * na + l1], X[jk * na + l1], v);
-        if (add)
-            out[b * Pb + i] += v;
-        else
-            out[b * Pb + i] = v;
+        out[b * Pb + i] = v;
+    }
+}
+
+// The downward change of degree for every lower degree at once: one
+// workgroup per (component, panel) interpolates the panel's potentials of
+// each lower degree that has far entries on it to the top degree, sums them
+// in registers and adds the sum into the top-degree array, which is read and
+// written once instead of once per degree.
+constexpr int kMaxLow = 3;  // lower degrees of a list (at most four in all)
+struct DegreeSet {
+    DegreeMatrix A[kMaxLow];        // nb x na[k], row-major
+    const double *in[kMaxLow];      // potentials of lower degree k, C * pct arrays
+    const int *goff[kMaxLow];       // that degree's group offsets: panels with entries
+    int na[kMaxLow];
+    int nlow;
+};
+
+__global__ void __launch_bounds__(TB)
+    k_degree_add(DegreeSet ds, int nb, double *__restrict__ out, int pct) {
+    __shared__ double X[kMaxP], Y[kMaxP];
+    const size_t b = blockIdx.x;
+    const int p = (int)(b % pct);
+    const int nb2 = nb * nb, Pb = nb2 * nb;
+    double acc[kOutPerThread];
+#pragma unroll
+    for (int r = 0; r < kOutPerThread; r++) acc[r] = 0.0;
+    bool any = false;
+    for (int k = 0; k < ds.nlow; k++) {
+        if (ds.goff[k][p + 1] == ds.goff[k][p]) continue;  // the same for the whole workgroup
+        any = true;
+        const int na = ds.na[k], a2 = na * na, Pa = a2 * na;
+        const double *A = ds.A[k].a;
+        const double *in = ds.in[k] + b * Pa;
+        for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb * a2; i += TB) {  // Y[j2][k1][l1] = sum_j1 A[j2][j1] X[j1][k1][l1]
+            const int j2 = i / a2, r = i % a2;
+            double v = 0.0;
+            for (int j1 = 0; j1 < na; j1++) v = fma(A[j2 * na + j1], X[j1 * a2 + r], v);
+            Y[i] = v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb2 * na; i += TB) {  // X[j2][k2][l1] = sum_k1 A[k2][k1] Y[j2][k1][l1]
+            const int j2 = i / (nb * na), k2 = (i / na) % nb, l1 = i % na;
+            double v = 0.0;
+            for (int k1 = 0; k1 < na; k1++) v = fma(A[k2 * na + k1], Y[j2 * a2 + k1 * na + l1], v);
+            X[i] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kOutPerThread; r++) {  // sum_l1 A[l2][l1] X[j2][k2][l1] into this thread's outputs
+            const int i = threadIdx.x + r * TB;
+            if (i < Pb) {
+                const int jk = i / nb, l2 = i % nb;
+                double v = 0.0;
+                for (int l1 = 0; l1 < na; l1++) v = fma(A[l2 * na + l1], X[jk * na + l1], v);
+                acc[r] += v;
+            }
+        }
+        __syncthreads();  // X is refilled by the next degree
+    }
+    if (!any) return;
+#pragma unroll
+    for (int r = 0; r < kOutPerThread; r++) {
+        const int i = threadIdx.x + r * TB;
+        if (i < Pb) out[b * Pb + i] += acc[r];
     }
 }
 
@@ -521,7 +577,7 @@ void upward_all(const btk_run_config &cfg, const List &l, const Tree &t, const d
         // W_d[k] = sum_m W_n[m] L_k(y_m): A[k][m], degree d out, degree n in
         const int d = l.deg[di];
         const DegreeMatrix A = transposed(degree_matrix(d, n), d, n);
-        k_degree_change<<<t.pc, TB, 0, s>>>(n + 1, d + 1, A, Wn, W + proxy_offset(l, t.pc, 1, di), 0, nullptr, 0);
+        k_degree_change<<<t.pc, TB, 0, s>>>(n + 1, d + 1, A, Wn, W + proxy_offset(l, t.pc, 1, di));
         BTK_LAUNCH_CHECK();
     }
 }
@@ -534,17 +590,22 @@ void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l
     double *Pn = Psi + proxy_offset(l, t.pc, C, top);
     if (top > 0 && t.pc > 0) {
         BTK_REQUIRE(t.dim == 3, "several interpolation degrees: 3D only");
+        BTK_REQUIRE(top <= kMaxLow, "at most four interpolation degrees");
+        BTK_REQUIRE(l.pct == t.pc, "the list's target panels are not this tree's");
+        // Psi_n[m] += sum_d sum_k L_k(y_m) Psi_d[k]: degree n out, each lower
+        // degree d in.  Only the far (CP, CC) entries of degree d write
+        // Psi_d, so a panel without such entries skips that degree.
+        DegreeSet ds{};
+        ds.nlow = top;
         for (int di = 0; di < top; di++) {
-            // Psi_n[m] += sum_k L_k(y_m) Psi_d[k]: degree n out, degree d in.
-            // Only the far (CP, CC) entries of degree d write Psi_d, so a
-            // panel without such entries is skipped.
             const int d = l.deg[di];
-            BTK_REQUIRE(l.pct == t.pc, "the list's target panels are not this tree's");
-            k_degree_change<<<C * t.pc, TB, 0, s>>>(d + 1, n + 1, degree_matrix(d, n),
-                                                    Psi + proxy_offset(l, t.pc, C, di), Pn, 1,
-                                                    l.goff.p + (size_t)(l.ndeg + di) * t.pc, t.pc);
-            BTK_LAUNCH_CHECK();
+            ds.A[di] = degree_matrix(d, n);
+            ds.in[di] = Psi + proxy_offset(l, t.pc, C, di);
+            ds.goff[di] = l.goff.p + (size_t)(l.ndeg + di) * t.pc;
+            ds.na[di] = d + 1;
         }
+        k_degree_add<<<C * t.pc, TB, 0, s>>>(ds, n + 1, Pn, t.pc);
+        BTK_LAUNCH_CHECK();
     }
     downward_pass(cfg, o, t, Pn, out, s, sorted_out);
 }
