@@ -218,7 +218,12 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             BTK_HIP(hipEventRecord(kev->ev[2], s));
         }
     }
-    if (p.w32.n) weights_to_f32(p.w32.n, W, p.w32.p, s);  // the CC products' A operand
+    for (int di = 0; di < p.ndeg; di++)  // the CC products' A operand
+        if (p.g[1][di].cc.n_items) {
+            const int P = ipow(p.deg[di] + 1, DIM);
+            weights_to_f16(p.ts->pc, P, cc_row_stride(P), W + p.woff[di], p.wh.p + p.w16off[di], p.wl.p + p.w16off[di],
+                           p.wexp.p + (size_t)di * p.ts->pc, s);
+        }
     for (int far = 0; far < 2; far++) {
         bool any = false;
         for (int di = 0; di < p.ndeg; di++) any |= p.g[far][di].present;
@@ -254,8 +259,9 @@ void launch_all(const InteractPlan &p, const double *q_sorted, const double *W, 
             L.comp_stride = far ? (size_t)tt.pc * P : (size_t)tt.n;
             if (g.mutual) launch_mutual<DIM, KERNEL>(g.mu, tt, q_sorted, p.kp, dst, p.sorted_out, s);
             if (g.cc.n_items)
-                launch_ccgemm(g.cc, tt, ts, p.w32.p + p.woff[di], n1, p.tabs[di], p.kp, KERNEL, dst, L.comp_stride,
-                              s);
+                launch_ccgemm(g.cc, tt, ts, p.wh.p + p.w16off[di], p.wl.p + p.w16off[di],
+                              p.wexp.p + (size_t)di * ts.pc, n1, p.tabs[di], p.kp, KERNEL, p.ccstage.p, dst,
+                              L.comp_stride, s);
             if (g.tk.n && !queued) launch_interact<DIM, KERNEL>(L, s);
         }
         if (kev && !queued) {
@@ -518,9 +524,22 @@ std::shared_ptr<InteractPlan> plan_interactions(const btk_run_config &cfg, const
             g.present = g.tk.n > 0 || g.mutual || g.cc.n_items > 0;
         }
     }
-    bool any_cc = false;
-    for (int di = 0; di < l.ndeg; di++) any_cc |= p->g[1][di].cc.n_items > 0;
-    if (any_cc) p->w32.resize(proxy_offset(l, ts.pc, 1, l.ndeg));
+    size_t n16 = 0, nstage = 0;
+    for (int di = 0; di < l.ndeg; di++) {
+        p->w16off[di] = n16;
+        const CCGemm &cc = p->g[1][di].cc;
+        if (cc.n_items > 0) {
+            const int P = ipow(l.deg[di] + 1, dim);
+            n16 += (size_t)ts.pc * cc_row_stride(P);
+            nstage = std::max(nstage, (size_t)cc.nv * C * cc_stage_stride(P));
+        }
+    }
+    if (nstage) p->ccstage.resize(nstage);
+    if (n16) {
+        p->wh.resize(n16);
+        p->wl.resize(n16);
+        p->wexp.resize((size_t)ts.pc * l.ndeg);
+    }
     if (o.schedule == BTK_SCHEDULE_QUEUE) build_queue(*p, l, tt, s);
     return p;
 }

@@ -559,18 +559,27 @@ extern template void launch_mutual<3, BTK_KERNEL_POT_GRAD>(const Mutual &, const
 // in meta, so the regular far kernel skips them.
 struct CCGemm {
     int n_items = 0;
+    int nv = 0;            // keyed entries (ent_T, ent_S)
     long long shared = 0;  // entries taken
     DBuf<int4> items;
     DBuf<int> ent_T, ent_S;  // target and source panel of each keyed entry, in key order
+    DBuf<int> tgt_off, tgt_k;  // per target panel, its keyed entries that items cover (CSR)
 };
 constexpr int kCcMinGroup = 8;
 void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,
                   CCGemm &cg, hipStream_t s);
-// W32: the source proxy weights of the group's degree rounded to FP32 (the
-// products' A operand), as written by weights_to_f32.
-void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W32, int n1, const NodeTables &tabs,
-                   const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s);
-void weights_to_f32(size_t n, const double *W, float *W32, hipStream_t s);
+// Split-FP16 source weights of one degree (the products' A operand): rows
+// of cc_row_stride(P) halves per source panel, hi and lo parts, and each
+// panel's power-of-two scale, as written by weights_to_f16.
+inline int cc_row_stride(int P) { return (P + 63) / 64 * 64; }
+// The products of each keyed entry are staged in FP32, C rows of
+// cc_stage_stride(P) floats, and added into Psi per target panel.
+inline int cc_stage_stride(int P) { return (P + 31) / 32 * 32; }
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const unsigned short *Wh,
+                   const unsigned short *Wl, const int *wexp, int n1, const NodeTables &tabs, const KParams &kp,
+                   int kernel, float *stage, double *Psi, size_t comp_stride, hipStream_t s);
+void weights_to_f16(int pc, int P, int Ps, const double *W, unsigned short *Wh, unsigned short *Wl, int *wexp,
+                    hipStream_t s);
 
 // Everything the interaction launches of one list need besides the charges,
 // the weights and the outputs: entry descriptors and the work items of the
@@ -597,7 +606,12 @@ struct InteractPlan {
     } g[2][4];  // [far][degree index]
     long long cc_shared = 0;  // CC entries evaluated through shared kernel matrices
     bool sorted_out = false;  // near output in the target tree's sorted order
-    DBuf<float> w32;          // FP32 copy of every degree's source weights, when a group has CC products
+    // split-FP16 source weights of the degrees with CC products (weights_to_f16):
+    // hi and lo rows, each panel's scale, and each degree's offsets in them
+    DBuf<unsigned short> wh, wl;
+    DBuf<int> wexp;
+    size_t w16off[4]{};
+    DBuf<float> ccstage;  // the products of one degree's keyed entries (launches of the degrees run in order)
     // btk_options.schedule = BTK_SCHEDULE_QUEUE: the work items of every
     // group as one queue, biggest first, drained by one persistent launch
     struct Queue {

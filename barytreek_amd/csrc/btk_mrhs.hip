@@ -427,86 +427,225 @@ __global__ void k_cc_entries(int nv, const int *__restrict__ vals, const int *__
     if (nitems[gid[k] - 1] > 0) meta[e].z = 0;
 }
 
+// Keyed entry k sorted by its target panel when an item covers it (its key
+// has work items), after every panel otherwise.
+__global__ void k_cc_taken_keys(int nv, const int *__restrict__ gid, const int *__restrict__ nitems,
+                                const int *__restrict__ entT, int pct, int *__restrict__ key, int *__restrict__ val) {
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= nv) return;
+    key[k] = nitems[gid[k] - 1] > 0 ? entT[k] : pct;
+    val[k] = k;
+}
+
+// off[p] = first position of sorted with sorted[.] >= p, p = 0 .. pct.
+__global__ void k_csr_offsets(int n, const int *__restrict__ sorted, int pct, int *__restrict__ off) {
+    const int p = blockIdx.x * TB + threadIdx.x;
+    if (p > pct) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sorted[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    off[p] = lo;
+}
+
+// Target panel p's potentials of components 0..C-1 += the staged products of
+// the keyed entries tk[off[p] .. off[p+1]), summed in FP64 in that order;
+// one workgroup per panel.
+__global__ void k_cc_reduce(int C, int P, int Pst, const int *__restrict__ off, const int *__restrict__ tk,
+                            const float *__restrict__ stage, double *__restrict__ out, size_t comp_stride) {
+    const int p = blockIdx.x;
+    const int e0 = off[p], e1 = off[p + 1];
+    if (e0 == e1) return;
+    for (int i = threadIdx.x; i < C * P; i += TB) {
+        const int c = i / P, t = i % P;
+        double sum = 0.0;
+        for (int e = e0; e < e1; e++) sum += (double)stage[((size_t)tk[e] * C + c) * Pst + t];
+        out[c * comp_stride + (size_t)p * P + t] += sum;
+    }
+}
+
+// Split-FP16 operands (Ootomo and Yokota, "Recovering single precision
+// accuracy from Tensor Cores while surpassing the FP32 theoretical peak
+// performance", IJHPCA 36(4), 2022): an FP32 value x travels as hi = fp16(x)
+// and lo = fp16(x - hi), both rounded to nearest, and a product as
+// hi_a hi_b + lo_a hi_b + hi_a lo_b; the dropped lo_a lo_b is below 2^-22 of
+// the product, and every fp16 product is exact in the FP32 accumulators.
+// Power-of-two scalings (exact) keep the operands below 2^14, inside the
+// fp16 range (largest finite value 65504) with room for rounding.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+constexpr int kCcExp = 14;
+
+__device__ __forceinline__ void split2(float a, float b, unsigned &hi, unsigned &lo) {
+    const half2v h = __builtin_convertvector((float2v){a, b}, half2v);
+    const half2v l = __builtin_convertvector((float2v){a - (float)h.x, b - (float)h.y}, half2v);
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, l);
+}
+
+__device__ __forceinline__ half8 as_half8(uint4 v) { return __builtin_bit_cast(half8, v); }
+
+// A workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// makes the workgroup's global writes visible, which waits for every
+// outstanding global store and atomic; the CC products' atomic adds go to
+// sums no thread of the launch reads, so their barriers need not wait.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Source proxy weights of one degree as split-FP16 rows, one wavefront per
+// source panel: the row is scaled by 2^e (e = wexp[panel]) so that its
+// largest magnitude lies in [2^13, 2^14), and padded with zeros to the row
+// stride Ps (a multiple of 64).
+__global__ void k_weights_f16(int pc, int P, int Ps, const double *__restrict__ W, unsigned short *__restrict__ Wh,
+                              unsigned short *__restrict__ Wl, int *__restrict__ wexp) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    if (c >= pc) return;
+    const double *w = W + (size_t)c * P;
+    double m = 0.0;
+    for (int j = lane; j < P; j += 64) m = fmax(m, fabs(w[j]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    int e = 0;
+    if (m > 0.0) {
+        int ex;
+        (void)frexp(m, &ex);  // 2^(ex-1) <= m < 2^ex
+        e = kCcExp - ex;
+    }
+    if (lane == 0) wexp[c] = e;
+    for (int j = 2 * lane; j < Ps; j += 128) {
+        const float a = j < P ? (float)ldexp(w[j], e) : 0.0f;
+        const float b = j + 1 < P ? (float)ldexp(w[j + 1], e) : 0.0f;
+        unsigned hi, lo;
+        split2(a, b, hi, lo);
+        *reinterpret_cast<unsigned *>(Wh + (size_t)c * Ps + j) = hi;
+        *reinterpret_cast<unsigned *>(Wl + (size_t)c * Ps + j) = lo;
+    }
+}
+
 // One work item: 128 target proxies x up to 32 pairs of one key, components
 // C0 .. C0+NC-1 of the kernel (every component in one launch: each kernel
-// value feeds NC products).  Wave w takes targets 32*w .. +32 and all 32 pairs and
-// forms the transposed product, pairs x targets: the A operand holds the
-// pairs' proxy weights (FP32, from LDS, loaded one tile ahead into
-// registers), the B operand the kernel of the key's first pair (FP32
-// coordinates relative to its target box centre, as the far kernel's
-// TileRel) -- lane l evaluates it for target l%32 and source parity l/32.
-// Summation is blocked in two FP32 levels: the matrix cores sum one 64-source
-// tile, the tile sums are added in FP32 registers (a pair's sum over P
-// proxies carries about 64 + P/64 roundings instead of P), and each pair's
-// sum is promoted to FP64 once, in the atomic add into its target cluster.
-// Each accumulator row is one pair, its lanes 32 consecutive target
-// proxies, so those FP64 atomic adds are coalesced.
-// Occupancy: the potential (one component) runs 3 wavefronts per SIMD, the
-// potential and gradient (four components, kernel values evaluated once for
-// all four products) 2 (250 VGPRs, no spills).
+// value feeds NC products).  Wave w takes targets 32*w .. +32 and all 32
+// pairs and forms the transposed product, pairs x targets, on fp16 matrix
+// cores (v_mfma_f32_32x32x16_f16, 16 sources per step, split-FP16 operands,
+// three products per step and component): the A operand holds the pairs'
+// proxy weights (split rows from k_weights_f16, staged in LDS one 64-source
+// tile at a time, the next tile's loads in flight), the B operand the kernel
+// of the key's first pair (FP32 coordinates relative to its target box
+// centre, as the far kernel's TileRel), lane l evaluating it for target l%32
+// and sources 8*(l/32) .. +8 of the step.  The fp16 products overlap the
+// kernel evaluations on the vector ALU.
+// The matrix cores sum a pair's sources in FP32 (3 * Pc / 16 accumulations,
+// 138 at degree 8); each pair's sums, unscaled, are stored to the pair's own
+// FP32 rows of `stage` (Pst floats per component, 128-byte aligned), and
+// k_cc_reduce adds the pairs of each target cluster in FP64.  Each
+// accumulator row is one pair, its lanes 32 consecutive target proxies, so
+// the stores are coalesced.  (FP64 atomic adds into the clusters' sums, one
+// per pair, proxy and component, ran at about 4e11 per second here: on the
+// 16M Plummer solve they cost 19 ms against 6 ms for the stores and the
+// reduction.)
+struct CcWeights {
+    const unsigned short *hi, *lo;  // [source panel][Ps] split-FP16 weights of one degree
+    const int *exp;                 // [source panel] their power-of-two scale
+    int stride;                     // Ps
+};
+
 template <int KERNEL, int C0, int NC>
-__global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
+__global__ void __launch_bounds__(TB, 2)
     k_cc_gemm(const int4 *__restrict__ items, const int *__restrict__ entT, const int *__restrict__ entS,
-              LevelBoxes TB_, LevelBoxes SB, const float *__restrict__ W, int n1, NodeTables tabs, KParams kp,
-              double *__restrict__ out, size_t comp_stride) {
+              LevelBoxes TB_, LevelBoxes SB, CcWeights wq, int n1, NodeTables tabs, KParams kp,
+              float *__restrict__ stage, int Pst) {
     constexpr int C = KernelTraits<3, KERNEL>::C;
-    constexpr int QPT = 64 * kCcCols / TB;  // weights per thread per tile
     static_assert(C0 + NC <= C, "component range");
     constexpr int kMaxP = kMaxN1 * kMaxN1 * kMaxN1;
-    __shared__ float sc[3][kMaxP];  // every source proxy of the key's first pair, relative to ctr
-    __shared__ float qt[64][kCcCols + 1];
-    __shared__ double cn[3][kMaxN1];
-    __shared__ int sS[kCcCols], sT[kCcCols];
+    constexpr int kMaxPs = (kMaxP + 63) / 64 * 64;
+    __shared__ __align__(16) float sc[3][kMaxPs];  // every source proxy of the key's first pair, relative to ctr
+    __shared__ float tc[3][kMaxP];                 // every target proxy of that pair, relative to ctr
+    __shared__ uint4 qs[2][kCcCols][9];  // hi, lo weights of the tile: 8 x 8 sources per pair, 1 of padding
+    __shared__ double cn[2][3][kMaxN1];  // source, target box nodes
+    __shared__ int sS[kCcCols], sT[kCcCols], sE[kCcCols];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int4 it = items[xcd_contiguous(blockIdx.x, gridDim.x)];
     const int gs = it.x, gn = it.y, p0 = it.w;
     const int T0 = entT[gs], S0 = entS[gs];
     const int P = n1 * n1 * n1;
+    const int Pc = (P + 15) & ~15;  // sources evaluated: whole 16-source steps
+    const int Ps = wq.stride;       // >= Pc, a multiple of 64
     const double ctr[3] = {0.5 * (TB_.lo0[T0] + TB_.hi0[T0]), 0.5 * (TB_.lo1[T0] + TB_.hi1[T0]),
                            0.5 * (TB_.lo2[T0] + TB_.hi2[T0])};
     const int col = lane & 31;
     const int par = lane >> 5;
     if (tid < kCcCols) {
-        sS[tid] = (p0 + tid < gn) ? entS[gs + p0 + tid] : -1;
-        sT[tid] = (p0 + tid < gn) ? entT[gs + p0 + tid] : -1;
+        const bool v = p0 + tid < gn;
+        const int S = v ? entS[gs + p0 + tid] : -1;
+        sS[tid] = S;
+        sT[tid] = v ? entT[gs + p0 + tid] : -1;
+        sE[tid] = wq.exp[S >= 0 ? S : S0];
     }
     if (tid < 3 * n1) {
         const int d = tid / n1, b = tid % n1;
-        cn[d][b] = proxy_node(tabs.b[b], sel3(d, SB.lo0[S0], SB.lo1[S0], SB.lo2[S0]),
-                              sel3(d, SB.hi0[S0], SB.hi1[S0], SB.hi2[S0]));
+        cn[0][d][b] = proxy_node(tabs.b[b], sel3(d, SB.lo0[S0], SB.lo1[S0], SB.lo2[S0]),
+                                 sel3(d, SB.hi0[S0], SB.hi1[S0], SB.hi2[S0]));
+        cn[1][d][b] = proxy_node(tabs.b[b], sel3(d, TB_.lo0[T0], TB_.lo1[T0], TB_.lo2[T0]),
+                                 sel3(d, TB_.hi0[T0], TB_.hi1[T0], TB_.hi2[T0]));
     }
-    __syncthreads();
-    // source proxy coordinates relative to the target box centre, once per
-    // item (no integer division by n1 in the tile loop)
-    for (int j = tid; j < P; j += TB) {
-        sc[0][j] = (float)(cn[0][j / (n1 * n1)] - ctr[0]);
-        sc[1][j] = (float)(cn[1][(j / n1) % n1] - ctr[1]);
-        sc[2][j] = (float)(cn[2][j % n1] - ctr[2]);
-    }
-    // this thread's share of a weight tile: source s = k % 64, pair r = k / 64.
-    // Every load is issued from an address in bounds and used as loaded, so
-    // the loads overlap the tile's products and are waited for only when the
-    // next tile is stored: a padding pair row reads the key's first weights
-    // (its output row is never written; rows of a product do not mix), a
-    // padding source the item's last proxy weight (its kernel values are
-    // zero, see step)
-    auto load_q = [&](int t0, float *q) {
+    // The boxes of a key are apart: every proxy pair is at least their gap g
+    // apart, and with 2^(eg-1) <= g the scaled kernel values 2^(kCcExp+eg-1)/r
+    // and 2^(kCcExp+2eg-2) |d|/r^3 stay below 2^kCcExp.
+    double g2 = 0.0;
 #pragma unroll
-        for (int i = 0; i < QPT; i++) {
-            const int k = tid + TB * i;
-            const int s = k % 64, r = k / 64;
-            const int S = sS[r];
-            q[i] = W[(size_t)(S >= 0 ? S : S0) * P + min(t0 + s, P - 1)];
+    for (int d = 0; d < 3; d++) {
+        const double tl = sel3(d, TB_.lo0[T0], TB_.lo1[T0], TB_.lo2[T0]), th = sel3(d, TB_.hi0[T0], TB_.hi1[T0], TB_.hi2[T0]);
+        const double sl = sel3(d, SB.lo0[S0], SB.lo1[S0], SB.lo2[S0]), sh = sel3(d, SB.hi0[S0], SB.hi1[S0], SB.hi2[S0]);
+        const double gap = fmax(0.0, fmax(sl - th, tl - sh));
+        g2 = fma(gap, gap, g2);
+    }
+    int eg;
+    (void)frexp(sqrt(g2), &eg);
+    const float fpot = ldexpf(1.0f, kCcExp + eg - 1), fgrad = ldexpf(1.0f, kCcExp + 2 * (eg - 1));
+    __syncthreads();
+    // source and target proxy coordinates relative to the target box
+    // centre, once per item (proxy_point's order); the padding sources
+    // P .. Pc (zero weights) sit at the last proxy
+    for (int j = tid; j < Pc; j += TB) {
+        const int jj = min(j, P - 1);
+        const int a = jj / (n1 * n1), b = (jj / n1) % n1, c = jj % n1;
+        sc[0][j] = (float)(cn[0][0][a] - ctr[0]);
+        sc[1][j] = (float)(cn[0][1][b] - ctr[1]);
+        sc[2][j] = (float)(cn[0][2][c] - ctr[2]);
+        if (j < P) {
+            tc[0][j] = (float)(cn[1][0][a] - ctr[0]);
+            tc[1][j] = (float)(cn[1][1][b] - ctr[1]);
+            tc[2][j] = (float)(cn[1][2][c] - ctr[2]);
         }
+    }
+    lds_barrier();
+    // this thread's share of a weight tile: pair r = tid / 8, sources 8 * (tid % 8) .. +8.
+    // A padding pair row reads the key's first weights (its output row is
+    // never written; rows of a product do not mix).
+    auto load_q = [&](int t0, uint4 &h, uint4 &l) {
+        const int S = sS[tid >> 3];
+        const size_t o = (size_t)(S >= 0 ? S : S0) * Ps + t0 + 8 * (tid & 7);
+        h = *reinterpret_cast<const uint4 *>(wq.hi + o);
+        l = *reinterpret_cast<const uint4 *>(wq.lo + o);
     };
     // the item runs every block of kCcTgt targets against every 64-source
     // tile, one sequence of (target block, source tile) steps; the weights
     // of the next step are loaded during the current one
-    const int nst = (P + 63) / 64;
+    const int nst = (Pc + 63) / 64;
     const int steps = nst * ((P + kCcTgt - 1) / kCcTgt);
-    float qn[QPT];
-    load_q(0, qn);
-    float acc[NC][16];
+    uint4 nh, nl;
+    load_q(0, nh, nl);
+    qs[0][tid >> 3][tid & 7] = nh;
+    qs[1][tid >> 3][tid & 7] = nl;
+    lds_barrier();
+    f32x16 d[NC];
     float tr[3] = {0.0f, 0.0f, 0.0f};
     int ti = 0;
     bool idle = false;  // every target of this wavefront is padding (P is not a multiple of 128)
@@ -514,93 +653,119 @@ __global__ void __launch_bounds__(TB, (NC == 1 ? 3 : 2))
     for (int st = 0; st < steps; st++) {
         const int t0 = (st % nst) * 64;
         if (t0 == 0) {  // a new block of targets
-            ti = (st / nst) * kCcTgt + 32 * w + (lane & 31);
+            ti = (st / nst) * kCcTgt + 32 * w + col;
             idle = (st / nst) * kCcTgt + 32 * w >= P;
-            double t[3];
-            proxy_point<3>(min(ti, P - 1), n1, tabs, TB_.lo0[T0], TB_.hi0[T0], TB_.lo1[T0], TB_.hi1[T0],
-                           TB_.lo2[T0], TB_.hi2[T0], t);
+            const int tj = min(ti, P - 1);
 #pragma unroll
-            for (int d = 0; d < 3; d++) tr[d] = (float)(t[d] - ctr[d]);
+            for (int dd = 0; dd < 3; dd++) tr[dd] = tc[dd][tj];
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) acc[c][v] = 0.0f;
+            for (int c = 0; c < NC; c++) d[c] = (f32x16){0};
         }
-        const int nt = min(64, P - t0);
-#pragma unroll
-        for (int i = 0; i < QPT; i++) {
-            const int k = tid + TB * i;
-            qt[k % 64][k / 64] = qn[i];
-        }
-        __syncthreads();
-        if (st + 1 < steps) load_q(((st + 1) % nst) * 64, qn);
-        f32x16 d[NC];
+        // the next tile's weights load while this one is evaluated
+        if (st + 1 < steps) load_q(((st + 1) % nst) * 64, nh, nl);
+        const int ns = min(4, (Pc - t0) >> 4);  // 16-source steps in this tile
         // The boxes of a CC pair are separated, so no kernel value needs the
-        // zero-distance test.  A full tile runs a fixed 32 steps (unrolled),
-        // a partial last one only the steps of its sources, where a padding
-        // source (at the item's last proxy) has zero kernel values.  The
-        // first step of a tile starts the FP32 sums from a zero operand.
-        auto step = [&](int k, auto first, bool clamp) {
-            const int s = 2 * k + par;
-            const int j = clamp ? min(t0 + s, P - 1) : t0 + s;
-            float g[C];
-            kernel_row<3, KERNEL, false>(tr[0] - sc[0][j], tr[1] - sc[1][j], tr[2] - sc[2][j], eps2, g);
-            if (clamp && t0 + s >= P) {
+        // zero-distance test.
+        auto step = [&](int k) {
+            const int j0 = t0 + 16 * k + 8 * par;
+            const half8 ah = as_half8(qs[0][col][2 * k + par]), al = as_half8(qs[1][col][2 * k + par]);
+            float xs[3][8];
 #pragma unroll
-                for (int c = 0; c < C; c++) g[c] = 0.0f;
+            for (int dd = 0; dd < 3; dd++) {
+                const float4 u = *reinterpret_cast<const float4 *>(&sc[dd][j0]);
+                const float4 v = *reinterpret_cast<const float4 *>(&sc[dd][j0 + 4]);
+                xs[dd][0] = u.x; xs[dd][1] = u.y; xs[dd][2] = u.z; xs[dd][3] = u.w;
+                xs[dd][4] = v.x; xs[dd][5] = v.y; xs[dd][6] = v.z; xs[dd][7] = v.w;
             }
-            const float q = qt[s][col];
+            uint4 bh[NC], bl[NC];
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-                d[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(q, g[C0 + c], decltype(first)::value ? (f32x16){0} : d[c],
-                                                            0, 0, 0);
-        };
-        if (!idle) {
-            if (nt == 64) {
-                step(0, std::true_type{}, false);
-#pragma unroll 4
-                for (int k = 1; k < 32; k++) step(k, std::false_type{}, false);
-            } else {
-                step(0, std::true_type{}, true);
-                for (int k = 1; k < (nt + 1) / 2; k++) step(k, std::false_type{}, true);
-            }
+            for (int mp = 0; mp < 4; mp++) {
+                float g[2][C];
 #pragma unroll
-            for (int c = 0; c < NC; c++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) acc[c][v] += d[c][v];
-        }
-        __syncthreads();
-        // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
-        if (t0 + 64 >= P && ti < P) {  // the block's last source tile: its sums are complete
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int Tc = sT[8 * (v / 4) + 4 * par + (v % 4)];
-                if (Tc < 0) continue;
+                for (int h = 0; h < 2; h++) {
+                    const int m = 2 * mp + h;
+                    const float dx = tr[0] - xs[0][m], dy = tr[1] - xs[1][m], dz = tr[2] - xs[2][m];
+                    const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(R);
+                    g[h][0] = inv * fpot;
+                    if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
+                        const float q3 = inv * inv * (inv * fgrad);
+                        g[h][1] = q3 * dx;
+                        g[h][2] = q3 * dy;
+                        g[h][3] = q3 * dz;
+                    }
+                }
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
-                    const double scale = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc);
-                    unsafeAtomicAdd(out + (C0 + c) * comp_stride + (size_t)Tc * P + ti, scale * (double)acc[c][v]);
+                    unsigned hi, lo;
+                    split2(g[0][C0 + c], g[1][C0 + c], hi, lo);
+                    (mp == 0 ? bh[c].x : mp == 1 ? bh[c].y : mp == 2 ? bh[c].z : bh[c].w) = hi;
+                    (mp == 0 ? bl[c].x : mp == 1 ? bl[c].y : mp == 2 ? bl[c].z : bl[c].w) = lo;
                 }
             }
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bh[c]), d[c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, as_half8(bh[c]), d[c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bl[c]), d[c], 0, 0, 0);
+        };
+        if (!idle) {
+            if (ns == 4) {
+                step(0);
+                step(1);
+                step(2);
+                step(3);
+            } else {
+                for (int k = 0; k < ns; k++) step(k);
+            }
         }
+        lds_barrier();
+        // The next tile goes to LDS before this block's stores are issued: a
+        // wait for a load also waits for every memory operation issued
+        // before it, so the stores are waited for only one tile later.
+        if (st + 1 < steps) {
+            qs[0][tid >> 3][tid & 7] = nh;
+            qs[1][tid >> 3][tid & 7] = nl;
+        }
+        // D[i][j]: pair row i = 8*(v/4) + 4*(lane/32) + v%4, target column j = lane%32
+        if (t0 + 64 >= Pc && ti < P) {  // the block's last source tile: its sums are complete
+            double us[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                us[c] = comp_scale_f32<3, KERNEL>(C0 + c, kp.gfc) / (double)(C0 + c == 0 ? fpot : fgrad);
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = 8 * (v / 4) + 4 * par + (v % 4);
+                if (sT[i] < 0) continue;
+                const int e = sE[i];
+                float *row = stage + (size_t)(gs + p0 + i) * C * Pst + ti;
+#pragma unroll
+                for (int c = 0; c < NC; c++) row[(size_t)(C0 + c) * Pst] = (float)ldexp((double)d[c][v] * us[c], -e);
+            }
+        }
+        lds_barrier();
     }
 }
 
 template <int KERNEL, int C0, int NC>
-void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1,
-                    const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
+void launch_cc_part(const CCGemm &cg, const Tree &tt, const Tree &ts, const CcWeights &wq, int n1,
+                    const NodeTables &tabs, const KParams &kp, float *stage, int Pst, hipStream_t s) {
     k_cc_gemm<KERNEL, C0, NC><<<cg.n_items, TB, 0, s>>>(cg.items.p, cg.ent_T.p, cg.ent_S.p, level_boxes(tt),
-                                                        level_boxes(ts), W, n1, tabs, kp, Psi, comp_stride);
+                                                        level_boxes(ts), wq, n1, tabs, kp, stage, Pst);
     BTK_LAUNCH_CHECK();
 }
 
 template <int KERNEL>
-void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1,
-                    const NodeTables &tabs, const KParams &kp, double *Psi, size_t comp_stride, hipStream_t s) {
+void launch_cc_gemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const CcWeights &wq, int n1,
+                    const NodeTables &tabs, const KParams &kp, float *stage, int Pst, hipStream_t s) {
     if constexpr (KernelTraits<3, KERNEL>::C == 1) {
-        launch_cc_part<KERNEL, 0, 1>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_part<KERNEL, 0, 1>(cg, tt, ts, wq, n1, tabs, kp, stage, Pst, s);
     } else {
-        launch_cc_part<KERNEL, 0, 4>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_part<KERNEL, 0, 4>(cg, tt, ts, wq, n1, tabs, kp, stage, Pst, s);
     }
 }
 
@@ -804,6 +969,19 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int mi
     BTK_LAUNCH_CHECK();
     k_cc_entries<<<nblk(nv), TB, 0, s>>>(nv, vals2.p, goff_g, pct, gid.p, nitems.p, meta, cg.ent_T.p, cg.ent_S.p);
     BTK_LAUNCH_CHECK();
+    {  // the covered entries of each target panel, for k_cc_reduce
+        DBuf<int> tk(nv), tk2(nv), kv(nv);
+        cg.tgt_k.resize(nv);
+        k_cc_taken_keys<<<nblk(nv), TB, 0, s>>>(nv, gid.p, nitems.p, cg.ent_T.p, pct, tk.p, kv.p);
+        BTK_LAUNCH_CHECK();
+        int pbits = 1;
+        while ((1ll << pbits) <= pct) pbits++;
+        sort_pairs_i32(tk.p, tk2.p, kv.p, cg.tgt_k.p, nv, pbits, s);
+        cg.tgt_off.resize((size_t)pct + 1);
+        k_csr_offsets<<<nblk(pct + 1), TB, 0, s>>>(nv, tk2.p, pct, cg.tgt_off.p);
+        BTK_LAUNCH_CHECK();
+    }
+    cg.nv = nv;
     {
         DBuf<int> ik(total), ik2(total), iv(total), iv2(total);
         DBuf<int4> unsorted(total);
@@ -819,26 +997,30 @@ void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int mi
     cg.n_items = total;
 }
 
-__global__ void k_weights_f32(size_t n, const double *__restrict__ W, float *__restrict__ W32) {
-    for (size_t i = (size_t)blockIdx.x * TB + threadIdx.x; i < n; i += (size_t)gridDim.x * TB) W32[i] = (float)W[i];
-}
-
-void weights_to_f32(size_t n, const double *W, float *W32, hipStream_t s) {
-    if (n == 0) return;
-    const int blocks = (int)std::min<size_t>((n + TB - 1) / TB, 8192);
-    k_weights_f32<<<blocks, TB, 0, s>>>(n, W, W32);
+void weights_to_f16(int pc, int P, int Ps, const double *W, unsigned short *Wh, unsigned short *Wl, int *wexp,
+                    hipStream_t s) {
+    if (pc == 0) return;
+    BTK_REQUIRE(Ps % 64 == 0 && Ps >= P, "split weight rows: a multiple of 64 at least P long");
+    k_weights_f16<<<(pc + TB / 64 - 1) / (TB / 64), TB, 0, s>>>(pc, P, Ps, W, Wh, Wl, wexp);
     BTK_LAUNCH_CHECK();
 }
 
-void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const float *W, int n1, const NodeTables &tabs,
-                   const KParams &kp, int kernel, double *Psi, size_t comp_stride, hipStream_t s) {
+void launch_ccgemm(const CCGemm &cg, const Tree &tt, const Tree &ts, const unsigned short *Wh,
+                   const unsigned short *Wl, const int *wexp, int n1, const NodeTables &tabs, const KParams &kp,
+                   int kernel, float *stage, double *Psi, size_t comp_stride, hipStream_t s) {
     if (cg.n_items == 0) return;
+    const int P = n1 * n1 * n1;
+    const CcWeights wq{Wh, Wl, wexp, cc_row_stride(P)};
+    const int Pst = cc_stage_stride(P);
     if (kernel == BTK_KERNEL_POT)
-        launch_cc_gemm<BTK_KERNEL_POT>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_gemm<BTK_KERNEL_POT>(cg, tt, ts, wq, n1, tabs, kp, stage, Pst, s);
     else if (kernel == BTK_KERNEL_POT_GRAD)
-        launch_cc_gemm<BTK_KERNEL_POT_GRAD>(cg, tt, ts, W, n1, tabs, kp, Psi, comp_stride, s);
+        launch_cc_gemm<BTK_KERNEL_POT_GRAD>(cg, tt, ts, wq, n1, tabs, kp, stage, Pst, s);
     else
         throw Error("shared-matrix CC products: potential or potential+gradient kernels");
+    const int C = kernel == BTK_KERNEL_POT ? 1 : 4;
+    k_cc_reduce<<<tt.pc, TB, 0, s>>>(C, P, Pst, cg.tgt_off.p, cg.tgt_k.p, stage, Psi, comp_stride);
+    BTK_LAUNCH_CHECK();
 }
 
 }  // namespace btk
