@@ -479,9 +479,13 @@ typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 constexpr int kCcExp = 14;
 
-__device__ __forceinline__ void split2(float a, float b, unsigned &hi, unsigned &lo) {
+// m1 = -1: the residuals x - hi (exact in FP32) are then fmas of an fp16
+// operand, one v_fma_mix_f32 each instead of a conversion and a subtraction;
+// the caller keeps m1 opaque, in a VGPR.
+__device__ __forceinline__ void split2(float a, float b, float m1, unsigned &hi, unsigned &lo) {
     const half2v h = __builtin_convertvector((float2v){a, b}, half2v);
-    const half2v l = __builtin_convertvector((float2v){a - (float)h.x, b - (float)h.y}, half2v);
+    const half2v l = __builtin_convertvector((float2v){__builtin_fmaf((float)h.x, m1, a), __builtin_fmaf((float)h.y, m1, b)},
+                                             half2v);
     hi = __builtin_bit_cast(unsigned, h);
     lo = __builtin_bit_cast(unsigned, l);
 }
@@ -523,7 +527,7 @@ __global__ void k_weights_f16(int pc, int P, int Ps, const double *__restrict__ 
         const float a = j < P ? (float)ldexp(w[j], e) : 0.0f;
         const float b = j + 1 < P ? (float)ldexp(w[j + 1], e) : 0.0f;
         unsigned hi, lo;
-        split2(a, b, hi, lo);
+        split2(a, b, -1.0f, hi, lo);
         *reinterpret_cast<unsigned *>(Wh + (size_t)c * Ps + j) = hi;
         *reinterpret_cast<unsigned *>(Wl + (size_t)c * Ps + j) = lo;
     }
@@ -650,6 +654,8 @@ __global__ void __launch_bounds__(TB, 2)
     int ti = 0;
     bool idle = false;  // every target of this wavefront is padding (P is not a multiple of 128)
     const float eps2 = (float)kp.eps2;
+    float m1 = -1.0f;
+    asm volatile("" : "+v"(m1));  // split2's -1 in a VGPR, not folded into subtractions
     for (int st = 0; st < steps; st++) {
         const int t0 = (st % nst) * 64;
         if (t0 == 0) {  // a new block of targets
@@ -669,49 +675,56 @@ __global__ void __launch_bounds__(TB, 2)
         auto step = [&](int k) {
             const int j0 = t0 + 16 * k + 8 * par;
             const half8 ah = as_half8(qs[0][col][2 * k + par]), al = as_half8(qs[1][col][2 * k + par]);
-            float xs[3][8];
+            // the kernel of the lane's target and its 8 sources: the
+            // distance part is shared by the components
+            float dx[8], dy[8], dz[8], ip[8], ig[8];
 #pragma unroll
-            for (int dd = 0; dd < 3; dd++) {
-                const float4 u = *reinterpret_cast<const float4 *>(&sc[dd][j0]);
-                const float4 v = *reinterpret_cast<const float4 *>(&sc[dd][j0 + 4]);
-                xs[dd][0] = u.x; xs[dd][1] = u.y; xs[dd][2] = u.z; xs[dd][3] = u.w;
-                xs[dd][4] = v.x; xs[dd][5] = v.y; xs[dd][6] = v.z; xs[dd][7] = v.w;
+            for (int h = 0; h < 2; h++) {
+                const float4 ux = *reinterpret_cast<const float4 *>(&sc[0][j0 + 4 * h]);
+                const float4 uy = *reinterpret_cast<const float4 *>(&sc[1][j0 + 4 * h]);
+                const float4 uz = *reinterpret_cast<const float4 *>(&sc[2][j0 + 4 * h]);
+                const float xs[3][4] = {{ux.x, ux.y, ux.z, ux.w}, {uy.x, uy.y, uy.z, uy.w}, {uz.x, uz.y, uz.z, uz.w}};
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    dx[4 * h + m] = tr[0] - xs[0][m];
+                    dy[4 * h + m] = tr[1] - xs[1][m];
+                    dz[4 * h + m] = tr[2] - xs[2][m];
+                }
             }
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const float R = fmaf(dx[m], dx[m], fmaf(dy[m], dy[m], fmaf(dz[m], dz[m], eps2)));
+                const float inv = __builtin_amdgcn_rsqf(R);
+                ip[m] = inv * fpot;
+                ig[m] = inv * inv * (inv * fgrad);
+            }
+            auto value = [&](int c, int m) {
+                return c == 0 ? ip[m] : ig[m] * (c == 1 ? dx[m] : c == 2 ? dy[m] : dz[m]);
+            };
+            // Stage q splits component q's values into the B operands and
+            // issues its hi*hi product, component q-1's lo*hi and q-2's
+            // hi*lo: the products of one stage run on the matrix cores while
+            // the next stage's values are formed on the vector ALU, and the
+            // three products into one accumulator are a stage apart.
             uint4 bh[NC], bl[NC];
 #pragma unroll
-            for (int mp = 0; mp < 4; mp++) {
-                float g[2][C];
+            for (int q = 0; q < NC + 2; q++) {
+                if (q < NC) {
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int m = 2 * mp + h;
-                    const float dx = tr[0] - xs[0][m], dy = tr[1] - xs[1][m], dz = tr[2] - xs[2][m];
-                    const float R = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(R);
-                    g[h][0] = inv * fpot;
-                    if constexpr (KERNEL == BTK_KERNEL_POT_GRAD) {
-                        const float q3 = inv * inv * (inv * fgrad);
-                        g[h][1] = q3 * dx;
-                        g[h][2] = q3 * dy;
-                        g[h][3] = q3 * dz;
+                    for (int mp = 0; mp < 4; mp++) {
+                        unsigned hi, lo;
+                        split2(value(C0 + q, 2 * mp), value(C0 + q, 2 * mp + 1), m1, hi, lo);
+                        (mp == 0 ? bh[q].x : mp == 1 ? bh[q].y : mp == 2 ? bh[q].z : bh[q].w) = hi;
+                        (mp == 0 ? bl[q].x : mp == 1 ? bl[q].y : mp == 2 ? bl[q].z : bl[q].w) = lo;
                     }
+                    d[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bh[q]), d[q], 0, 0, 0);
                 }
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    unsigned hi, lo;
-                    split2(g[0][C0 + c], g[1][C0 + c], hi, lo);
-                    (mp == 0 ? bh[c].x : mp == 1 ? bh[c].y : mp == 2 ? bh[c].z : bh[c].w) = hi;
-                    (mp == 0 ? bl[c].x : mp == 1 ? bl[c].y : mp == 2 ? bl[c].z : bl[c].w) = lo;
-                }
+                if (q >= 1 && q - 1 < NC)
+                    d[q - 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, as_half8(bh[q - 1]), d[q - 1], 0, 0, 0);
+                if (q >= 2)
+                    d[q - 2] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bl[q - 2]), d[q - 2], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
             }
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bh[c]), d[c], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, as_half8(bh[c]), d[c], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-                d[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, as_half8(bl[c]), d[c], 0, 0, 0);
         };
         if (!idle) {
             if (ns == 4) {

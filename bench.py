@@ -127,10 +127,11 @@ def parse():
                     help="learned MAC: this per-pair budget instead of the held-out calibration (for profiling)")
     ap.add_argument("--mac-target", type=float, default=0.85e-6,
                     help="learned MAC: rel-L2 the calibration asks of every held-out set (margin below 1e-6)")
-    ap.add_argument("--mac-costs", default="1,1,0.8,0.2", metavar="PP,PC,CP,CC",
+    ap.add_argument("--mac-costs", default="1,1,0.8,0.12", metavar="PP,PC,CP,CC",
                     help="learned MAC: relative cost of one kernel evaluation of each pair type (default: measured "
                          "on the 16M solve, a near evaluation 0.40 ns, CP and unshared CC on the vector ALU 0.32 ns "
-                         "with two targets per lane, a CC evaluation as a shared-matrix MFMA product 0.09 ns)")
+                         "with two targets per lane, a CC evaluation as a shared-matrix split-FP16 MFMA product with its "
+                         "staging and reduction 0.048 ns)")
     ap.add_argument("--mac-holdout", type=int, default=3, help="learned MAC: held-out sets for the calibration")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES]",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")

@@ -366,10 +366,10 @@ int btk_mac_set_budget(double rel_err);
 /* Relative cost of one kernel evaluation of a PP, PC, CP and CC pair
  * (costs[0..3], each > 0; default all 1, so that the learned MAC minimises
  * evaluations).  The learned MAC takes, among the candidates within the
- * budget, the one of least weighted cost: with CC products on MFMA
- * (mixed precision, btk_tuning.cc_group) a CC evaluation costs about a
- * quarter of a near-field one on the vector ALU (MI355X, 3D potential and
- * gradient: 0.09 against 0.40 ns).  Process-wide, like the budget. */
+ * budget, the one of least weighted cost: with CC products on the matrix
+ * cores (mixed precision, btk_tuning.cc_group) a CC evaluation costs about
+ * an eighth of a near-field one on the vector ALU (MI355X, 3D potential and
+ * gradient: 0.048 against 0.40 ns).  Process-wide, like the budget. */
 int btk_mac_set_costs(const float *costs);
 /* The installed network on d_X[n][10]: d_out[n][12] (scaled outputs). */
 int btk_mac_forward(const float *d_X, int n, float *d_out, void *stream);
