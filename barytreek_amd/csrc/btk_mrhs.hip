@@ -451,18 +451,38 @@ __global__ void k_csr_offsets(int n, const int *__restrict__ sorted, int pct, in
 }
 
 // Target panel p's potentials of components 0..C-1 += the staged products of
-// the keyed entries tk[off[p] .. off[p+1]), summed in FP64 in that order;
-// one workgroup per panel.
-__global__ void k_cc_reduce(int C, int P, int Pst, const int *__restrict__ off, const int *__restrict__ tk,
-                            const float *__restrict__ stage, double *__restrict__ out, size_t comp_stride) {
+// the keyed entries tk[off[p] .. off[p+1]), summed in FP64 in that order
+// (per chunk of kRedChunk entries); one workgroup per panel.  The entries'
+// indices sit in LDS, and each thread issues the loads of 8 entries together
+// before adding them in order.
+constexpr int kRedChunk = 1024;
+__global__ void __launch_bounds__(TB)
+    k_cc_reduce(int C, int P, int Pst, const int *__restrict__ off, const int *__restrict__ tk,
+                const float *__restrict__ stage, double *__restrict__ out, size_t comp_stride) {
+    __shared__ int sk[kRedChunk];
     const int p = blockIdx.x;
     const int e0 = off[p], e1 = off[p + 1];
     if (e0 == e1) return;
-    for (int i = threadIdx.x; i < C * P; i += TB) {
-        const int c = i / P, t = i % P;
-        double sum = 0.0;
-        for (int e = e0; e < e1; e++) sum += (double)stage[((size_t)tk[e] * C + c) * Pst + t];
-        out[c * comp_stride + (size_t)p * P + t] += sum;
+    const size_t rs = (size_t)C * Pst;  // one entry's staged floats
+    for (int c0 = e0; c0 < e1; c0 += kRedChunk) {
+        const int ne = min(kRedChunk, e1 - c0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < ne; e += TB) sk[e] = tk[c0 + e];
+        __syncthreads();
+        for (int i = threadIdx.x; i < C * P; i += TB) {
+            const float *col = stage + (i / P) * Pst + i % P;
+            double sum = 0.0;
+            int e = 0;
+            for (; e + 8 <= ne; e += 8) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = col[(size_t)sk[e + u] * rs];
+#pragma unroll
+                for (int u = 0; u < 8; u++) sum += (double)v[u];
+            }
+            for (; e < ne; e++) sum += (double)col[(size_t)sk[e] * rs];
+            out[(i / P) * comp_stride + (size_t)p * P + i % P] += sum;
+        }
     }
 }
 

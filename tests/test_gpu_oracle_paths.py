@@ -123,6 +123,44 @@ def test_solves_vs_oracle(kind, n, dim, deg, leaf, theta, eps, kernel, excl):
             assert st.cc_shared > 0.3 * cnt[3]
 
 
+@pytest.mark.parametrize("se,ce", [(-16, -30), (16, 30), (-16, 30), (16, -30)])
+def test_shared_cc_scale_invariance(se, ce):
+    """The shared-matrix CC products carry their operands as split fp16 with
+    power-of-two scalings chosen from the boxes' gap and each cluster's
+    largest weight.  Coordinates scaled by 2^se and charges by 2^ce (exact in
+    FP64: the same tree and lists) must give the oracle's solution scaled by
+    2^(ce-se) (potential) and 2^(ce-2se) (gradient), each component as close
+    as at scale 1."""
+    kind, n, dim, deg, leaf, theta, eps, kernel, excl = CASES[4]
+    print(f"\n[oracle paths] shared CC at coordinates x 2^{se}, charges x 2^{ce}: {kind} n={n}", flush=True)
+    seed = 500 + n + deg
+    ref, cnt = oracle_solve(kind, n, seed, dim, deg, leaf, theta, eps, kernel, excl)
+    C = btk.components(dim, kernel)
+    ref = np.asarray(ref).reshape(C, n)
+    x, y, z, q = points(kind, n, seed)
+    cfg = btk.make_config(dim, deg, leaf, theta, eps)
+    btk.set_tuning(cc_group=2)
+
+    def run(s, c):
+        d = [torch.from_numpy(np.ascontiguousarray(a * s)).cuda() for a in (x, y, z)]
+        out = torch.zeros(C * n, dtype=torch.float64, device="cuda")
+        st = btk.solve(cfg, btk.options(kernel=kernel, exclude_self=excl, precision=btk.PREC_MIXED), d[0], d[1],
+                       d[2], torch.from_numpy(q * c).cuda(), out)
+        assert [int(st.count[k]) for k in range(4)] == cnt
+        assert st.cc_shared > 0.3 * cnt[3]
+        return out.view(C, n).cpu().numpy()
+
+    base = run(1.0, 1.0)
+    s, c = 2.0 ** se, 2.0 ** ce
+    got = run(s, c)
+    want = ref * np.array([c / s] + [c / (s * s)] * (C - 1))[:, None]
+    for k in range(C):
+        e0, e = rel(base[k], ref[k]), rel(got[k], want[k])
+        print(f"  component {k}: vs oracle {e:.2e} (scale 1: {e0:.2e})", flush=True)
+        assert e < max(2.0 * e0, 1e-9), k
+    assert rel(got, want) < MIXED_TOL
+
+
 MULTI = [
     # kind, n, dim, degree, leaf, theta, eps, kernel, exclude_self, vectors
     ("cube3", 20000, 3, 6, 300, 0.7, 0.0, btk.KERNEL_POT, 1, 64),
