@@ -280,19 +280,27 @@ __global__ void k_add_unsorted(int n, long long total, const int *__restrict__ p
 // ---------------------------------------------------------------------------
 // leaf -> point (downward_pass_impl.hpp:207-236): one workgroup per leaf.
 // ---------------------------------------------------------------------------
-template <int DIM, int N1>
+// Leaf -> point: one workgroup per leaf, one target per thread; the leaf's
+// potentials sit in LDS proxy by proxy (NC components of one proxy together,
+// padded to CP), so each read feeds one independent FMA chain per component
+// and every offset fits one base register's immediate.  Sum-factorised:
+// (n+1)^dim + (n+1)^(dim-1) + ... FMAs per component and target.
+template <int DIM, int N1, int NC>
 __global__ void __launch_bounds__(TB)
     k_l2p(const int *__restrict__ leaves, const int *__restrict__ start, const int *__restrict__ count, Boxes bx,
           const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
-          const int *__restrict__ perm, NodeTables tabs, int ncomp, size_t comp_stride,
-          const double *__restrict__ Psi, double *__restrict__ out, int M) {
+          const int *__restrict__ perm, NodeTables tabs, size_t comp_stride, const double *__restrict__ Psi,
+          double *__restrict__ out, int M) {
     constexpr int P = DIM == 3 ? N1 * N1 * N1 : N1 * N1;
-    __shared__ double ps[4][P];
+    constexpr int CP = NC == 3 ? 4 : NC;
+    __shared__ __align__(16) double ps[P][CP];
     const int leaf = leaves[blockIdx.x];
     const int s0 = start[leaf], cnt = count[leaf];
     if (cnt == 0) return;
-    for (int c = 0; c < ncomp; c++)
-        for (int idx = threadIdx.x; idx < P; idx += TB) ps[c][idx] = Psi[c * comp_stride + (size_t)leaf * P + idx];
+    for (int idx = threadIdx.x; idx < P * NC; idx += TB) {
+        const int c = idx / P, pp = idx % P;
+        ps[pp][c] = Psi[c * comp_stride + (size_t)leaf * P + pp];
+    }
     __syncthreads();
     const double lo0 = bx.lo0[leaf], hi0 = bx.hi0[leaf], lo1 = bx.lo1[leaf], hi1 = bx.hi1[leaf];
     for (int i = threadIdx.x; i < cnt; i += TB) {
@@ -301,32 +309,54 @@ __global__ void __launch_bounds__(TB)
         bli_basis<N1>(sy[s0 + i], lo1, hi1, tabs.a, N1, Ly);
         if constexpr (DIM == 3) bli_basis<N1>(sz[s0 + i], bx.lo2[leaf], bx.hi2[leaf], tabs.a, N1, Lz);
         const int o = perm ? perm[s0 + i] : s0 + i;
-        for (int c = 0; c < ncomp; c++) {
-            double phi = 0.0;
-            if constexpr (DIM == 3) {
+        double phi[NC];
 #pragma unroll
-                for (int j = 0; j < N1; j++) {
-                    double sj = 0.0;
+        for (int c = 0; c < NC; c++) phi[c] = 0.0;
+        if constexpr (DIM == 3) {
 #pragma unroll
-                    for (int k = 0; k < N1; k++) {
-                        double sk = 0.0;
+            for (int j = 0; j < N1; j++) {
+                double sj[NC];
 #pragma unroll
-                        for (int l = 0; l < N1; l++) sk = fma(ps[c][(j * N1 + k) * N1 + l], Lz[l], sk);
-                        sj = fma(sk, Ly[k], sj);
+                for (int c = 0; c < NC; c++) sj[c] = 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; k++) {
+                    double sk[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) sk[c] = 0.0;
+#pragma unroll
+                    for (int l = 0; l < N1; l++) {
+                        const double *v = ps[(j * N1 + k) * N1 + l];
+#pragma unroll
+                        for (int c = 0; c < NC; c++) sk[c] = fma(v[c], Lz[l], sk[c]);
                     }
-                    phi = fma(sj, Lx[j], phi);
-                }
-            } else {
 #pragma unroll
-                for (int j = 0; j < N1; j++) {
-                    double sj = 0.0;
-#pragma unroll
-                    for (int k = 0; k < N1; k++) sj = fma(ps[c][j * N1 + k], Ly[k], sj);
-                    phi = fma(sj, Lx[j], phi);
+                    for (int c = 0; c < NC; c++) sj[c] = fma(sk[c], Ly[k], sj[c]);
                 }
+#pragma unroll
+                for (int c = 0; c < NC; c++) phi[c] = fma(sj[c], Lx[j], phi[c]);
+                // one slab of the grid at a time: the scheduler hoists LDS
+                // reads only within it (registers, no spills)
+                __builtin_amdgcn_sched_barrier(0);
             }
-            out[(size_t)c * M + o] += phi;
+        } else {
+#pragma unroll
+            for (int j = 0; j < N1; j++) {
+                double sj[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) sj[c] = 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; k++) {
+                    const double *v = ps[j * N1 + k];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) sj[c] = fma(v[c], Ly[k], sj[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; c++) phi[c] = fma(sj[c], Lx[j], phi[c]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
+#pragma unroll
+        for (int c = 0; c < NC; c++) out[(size_t)c * M + o] += phi[c];
     }
 }
 
@@ -347,13 +377,13 @@ void launch_anterp(int n1, int nleaves, const Tree &t, const double *q, const No
     BTK_LAUNCH_CHECK();
 }
 
-template <int DIM>
-void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const NodeTables &tabs, int ncomp,
-                size_t stride, const int *perm, const double *Psi, double *out, hipStream_t s) {
+template <int DIM, int NC>
+void launch_l2p_nc(int n1, const int *leaves, int nleaves, const Tree &t, const NodeTables &tabs, size_t stride,
+                   const int *perm, const double *Psi, double *out, hipStream_t s) {
 #define BTK_L2P(N)                                                                                              \
     case N:                                                                                                     \
-        k_l2p<DIM, N><<<nleaves, TB, 0, s>>>(leaves, t.start.p, t.count.p, boxes(t), t.sx.p, t.sy.p, t.sz.p,    \
-                                             perm, tabs, ncomp, stride, Psi, out, t.n);                         \
+        k_l2p<DIM, N, NC><<<nleaves, TB, 0, s>>>(leaves, t.start.p, t.count.p, boxes(t), t.sx.p, t.sy.p,        \
+                                                 t.sz.p, perm, tabs, stride, Psi, out, t.n);                    \
         break;
     switch (n1) {
         BTK_L2P(2) BTK_L2P(3) BTK_L2P(4) BTK_L2P(5) BTK_L2P(6) BTK_L2P(7) BTK_L2P(8) BTK_L2P(9) BTK_L2P(10)
@@ -362,6 +392,17 @@ void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const Nod
     }
 #undef BTK_L2P
     BTK_LAUNCH_CHECK();
+}
+
+template <int DIM>
+void launch_l2p(int n1, const int *leaves, int nleaves, const Tree &t, const NodeTables &tabs, int ncomp,
+                size_t stride, const int *perm, const double *Psi, double *out, hipStream_t s) {
+    switch (ncomp) {
+        case 1: launch_l2p_nc<DIM, 1>(n1, leaves, nleaves, t, tabs, stride, perm, Psi, out, s); break;
+        case 3: launch_l2p_nc<DIM, 3>(n1, leaves, nleaves, t, tabs, stride, perm, Psi, out, s); break;
+        case 4: launch_l2p_nc<DIM, 4>(n1, leaves, nleaves, t, tabs, stride, perm, Psi, out, s); break;
+        default: throw Error("leaf-to-point: 1, 3 or 4 components");
+    }
 }
 
 // ---------------------------------------------------------------------------
