@@ -565,7 +565,7 @@ struct CCGemm {
     DBuf<int> ent_T, ent_S;  // target and source panel of each keyed entry, in key order
     DBuf<int> tgt_off, tgt_k;  // per target panel, its keyed entries that items cover (CSR)
 };
-constexpr int kCcMinGroup = 8;
+constexpr int kCcMinGroup = 4;
 void build_ccgemm(const List &l, const Tree &tt, const Tree &ts, int grp, int min_group, int4 *meta,
                   CCGemm &cg, hipStream_t s);
 // Split-FP16 source weights of one degree (the products' A operand): rows

@@ -133,6 +133,9 @@ def parse():
                          "with two targets per lane, a CC evaluation as a shared-matrix split-FP16 MFMA product with its "
                          "staging and reduction 0.048 ns)")
     ap.add_argument("--mac-holdout", type=int, default=3, help="learned MAC: held-out sets for the calibration")
+    ap.add_argument("--cc-group", type=int, default=0,
+                    help="minimum CC pairs sharing a kernel matrix that run as one matrix-core product "
+                         "(btk_tuning.cc_group; 0 = the library default, 4; -1 = off)")
     ap.add_argument("--shapes", default=None, metavar="NEAR_ITEMS,FAR_ITEMS,NEAR_TPT,FAR_TPT,MUTUAL[,NEAR_WAVES,FAR_WAVES]",
                     help="launch shapes to use instead of autotuning (e.g. the ones a previous run chose)")
     ap.add_argument("--cpu-stride", type=int, default=10, help="CPU baseline up to 1M points: every k-th block of 64 PP/PC pairs")
@@ -714,9 +717,10 @@ def main():
                 budget, choice = calibrate_budget(btk, torch, spec, n, cfg, opts, args.mac_target, args.mac_holdout,
                                                   dev, args.sample_targets)
             btk.mac_set_budget(budget)
-        btk.set_tuning()  # defaults
+        group = {"cc_group": args.cc_group} if args.cc_group else {}
+        btk.set_tuning(**group)  # defaults, and the CC group size when given
         if args.shapes:
-            btk.set_tuning(**shape_args(args.shapes))
+            btk.set_tuning(**shape_args(args.shapes), **group)
         elif not args.no_autotune:  # untimed, like the warmup
             log(f"autotune {v}")
             btk.autotune(cfg, opts, d[0], d[1], d[2], d[3])
@@ -732,7 +736,8 @@ def main():
                           mac_costs=dict(zip(("pp", "pc", "cp", "cc"), mac_costs)))
         res[v]["launch_shapes"] = {"near_items": t.near_items, "far_items": t.far_items, "near_tpt": t.near_tpt,
                                    "far_tpt": t.far_tpt, "mutual": t.mutual, "near_waves": t.near_waves,
-                                   "far_waves": t.far_waves, "autotuned": not (args.no_autotune or args.shapes)}
+                                   "far_waves": t.far_waves, "cc_group": t.cc_group,
+                                   "autotuned": not (args.no_autotune or args.shapes)}
         log(f"{v}: {ms:.2f} ms/solve, rel-L2 {res[v].get('rel_l2')}")
     # the fastest variant that meets the metric's error
     meets = [v for v in res if ds is None or res[v]["rel_l2"] <= 1e-6]

@@ -144,7 +144,7 @@ typedef struct btk_tuning {
     int cc_group;   /* mixed precision, 3D, potential (+gradient): CC pairs with the same two levels and
                        box-centre offset share their kernel matrix; a set of at least cc_group such pairs
                        runs as one matrix-core product (split-fp16 operands, FP32 accuracy per product;
-                       each target panel's pairs summed in FP64 in a fixed order).  0 = default (8),
+                       each target panel's pairs summed in FP64 in a fixed order).  0 = default (4),
                        -1 = off */
     int near_waves; /* wavefronts per workgroup of the PP+PC kernel, 1, 2 or 4 (0 = default: 4 in 3D, 1 in
                        2D): they take
