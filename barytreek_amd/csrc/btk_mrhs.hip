@@ -633,20 +633,24 @@ __global__ void __launch_bounds__(TB, 2)
     int eg;
     (void)frexp(sqrt(g2), &eg);
     const float fpot = ldexpf(1.0f, kCcExp + eg - 1), fgrad = ldexpf(1.0f, kCcExp + 2 * (eg - 1));
+    // The coordinates carry the potential's scale: with d' = d / fpot,
+    // 1/|d'| = fpot/|d| and |d'|/|d'|^3 = 2^kCcExp * fgrad |d|/|d|^3 (powers of
+    // two, exact), so the potential needs no product and the gradient one.
+    const double sig = ldexp(1.0, -(kCcExp + eg - 1));
     __syncthreads();
     // source and target proxy coordinates relative to the target box
-    // centre, once per item (proxy_point's order); the padding sources
-    // P .. Pc (zero weights) sit at the last proxy
+    // centre, scaled by sig, once per item (proxy_point's order); the
+    // padding sources P .. Pc (zero weights) sit at the last proxy
     for (int j = tid; j < Pc; j += TB) {
         const int jj = min(j, P - 1);
         const int a = jj / (n1 * n1), b = (jj / n1) % n1, c = jj % n1;
-        sc[0][j] = (float)(cn[0][0][a] - ctr[0]);
-        sc[1][j] = (float)(cn[0][1][b] - ctr[1]);
-        sc[2][j] = (float)(cn[0][2][c] - ctr[2]);
+        sc[0][j] = (float)((cn[0][0][a] - ctr[0]) * sig);
+        sc[1][j] = (float)((cn[0][1][b] - ctr[1]) * sig);
+        sc[2][j] = (float)((cn[0][2][c] - ctr[2]) * sig);
         if (j < P) {
-            tc[0][j] = (float)(cn[1][0][a] - ctr[0]);
-            tc[1][j] = (float)(cn[1][1][b] - ctr[1]);
-            tc[2][j] = (float)(cn[1][2][c] - ctr[2]);
+            tc[0][j] = (float)((cn[1][0][a] - ctr[0]) * sig);
+            tc[1][j] = (float)((cn[1][1][b] - ctr[1]) * sig);
+            tc[2][j] = (float)((cn[1][2][c] - ctr[2]) * sig);
         }
     }
     lds_barrier();
@@ -673,7 +677,8 @@ __global__ void __launch_bounds__(TB, 2)
     float tr[3] = {0.0f, 0.0f, 0.0f};
     int ti = 0;
     bool idle = false;  // every target of this wavefront is padding (P is not a multiple of 128)
-    const float eps2 = (float)kp.eps2;
+    const float eps2 = (float)(kp.eps2 * sig * sig);
+    const float gs14 = ldexpf(1.0f, -kCcExp);
     float m1 = -1.0f;
     asm volatile("" : "+v"(m1));  // split2's -1 in a VGPR, not folded into subtractions
     for (int st = 0; st < steps; st++) {
@@ -715,8 +720,8 @@ __global__ void __launch_bounds__(TB, 2)
             for (int m = 0; m < 8; m++) {
                 const float R = fmaf(dx[m], dx[m], fmaf(dy[m], dy[m], fmaf(dz[m], dz[m], eps2)));
                 const float inv = __builtin_amdgcn_rsqf(R);
-                ip[m] = inv * fpot;
-                ig[m] = inv * inv * (inv * fgrad);
+                ip[m] = inv;
+                ig[m] = inv * inv * (inv * gs14);
             }
             auto value = [&](int c, int m) {
                 return c == 0 ? ip[m] : ig[m] * (c == 1 ? dx[m] : c == 2 ? dy[m] : dz[m]);

@@ -458,53 +458,81 @@ struct DegreeSet {
     int nlow;
 };
 
+// One lower degree's contribution, sizes known at compile time: X holds the
+// panel's degree-(NA-1) potentials, A (NB x NA, in LDS) interpolates each
+// axis; acc[r] += the degree-(NB-1) values of this thread's outputs.
+template <int NA, int NB>
+__device__ __forceinline__ void degree_add_one(const double *__restrict__ in, const double *A, double *X, double *Y,
+                                               double *acc) {
+    constexpr int a2 = NA * NA, Pa = a2 * NA, nb2 = NB * NB, Pb = nb2 * NB;
+    constexpr int R = (Pb + TB - 1) / TB;
+    for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < NB * a2; i += TB) {  // Y[j2][k1][l1] = sum_j1 A[j2][j1] X[j1][k1][l1]
+        const int j2 = i / a2, r = i % a2;
+        double v = 0.0;
+#pragma unroll
+        for (int j1 = 0; j1 < NA; j1++) v = fma(A[j2 * NA + j1], X[j1 * a2 + r], v);
+        Y[i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb2 * NA; i += TB) {  // X[j2][k2][l1] = sum_k1 A[k2][k1] Y[j2][k1][l1]
+        const int j2 = i / (NB * NA), k2 = (i / NA) % NB, l1 = i % NA;
+        double v = 0.0;
+#pragma unroll
+        for (int k1 = 0; k1 < NA; k1++) v = fma(A[k2 * NA + k1], Y[j2 * a2 + k1 * NA + l1], v);
+        X[i] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {  // sum_l1 A[l2][l1] X[j2][k2][l1] into this thread's outputs
+        const int i = threadIdx.x + r * TB;
+        if (i < Pb) {
+            const int jk = i / NB, l2 = i % NB;
+            double v = 0.0;
+#pragma unroll
+            for (int l1 = 0; l1 < NA; l1++) v = fma(A[l2 * NA + l1], X[jk * NA + l1], v);
+            acc[r] += v;
+        }
+    }
+    __syncthreads();  // X is refilled by the next degree
+}
+
+template <int NA, int NB>
+__device__ __forceinline__ void degree_add_dispatch(int na, const double *in, const double *A, double *X, double *Y,
+                                                    double *acc) {
+    if constexpr (NA < NB) {
+        if (na == NA)
+            degree_add_one<NA, NB>(in, A, X, Y, acc);
+        else
+            degree_add_dispatch<NA + 1, NB>(na, in, A, X, Y, acc);
+    }
+}
+
+template <int NB>
 __global__ void __launch_bounds__(TB)
-    k_degree_add(DegreeSet ds, int nb, double *__restrict__ out, int pct) {
+    k_degree_add(DegreeSet ds, double *__restrict__ out, int pct) {
+    constexpr int Pb = NB * NB * NB;
+    constexpr int R = (Pb + TB - 1) / TB;
     __shared__ double X[kMaxP], Y[kMaxP];
+    __shared__ double sA[kMaxLow][kMaxN1 * kMaxN1];
     const size_t b = blockIdx.x;
     const int p = (int)(b % pct);
-    const int nb2 = nb * nb, Pb = nb2 * nb;
-    double acc[kOutPerThread];
+    for (int k = 0; k < ds.nlow; k++)
+        for (int i = threadIdx.x; i < NB * ds.na[k]; i += TB) sA[k][i] = ds.A[k].a[i];
+    double acc[R];
 #pragma unroll
-    for (int r = 0; r < kOutPerThread; r++) acc[r] = 0.0;
+    for (int r = 0; r < R; r++) acc[r] = 0.0;
     bool any = false;
     for (int k = 0; k < ds.nlow; k++) {
         if (ds.goff[k][p + 1] == ds.goff[k][p]) continue;  // the same for the whole workgroup
         any = true;
-        const int na = ds.na[k], a2 = na * na, Pa = a2 * na;
-        const double *A = ds.A[k].a;
-        const double *in = ds.in[k] + b * Pa;
-        for (int i = threadIdx.x; i < Pa; i += TB) X[i] = in[i];
-        __syncthreads();
-        for (int i = threadIdx.x; i < nb * a2; i += TB) {  // Y[j2][k1][l1] = sum_j1 A[j2][j1] X[j1][k1][l1]
-            const int j2 = i / a2, r = i % a2;
-            double v = 0.0;
-            for (int j1 = 0; j1 < na; j1++) v = fma(A[j2 * na + j1], X[j1 * a2 + r], v);
-            Y[i] = v;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < nb2 * na; i += TB) {  // X[j2][k2][l1] = sum_k1 A[k2][k1] Y[j2][k1][l1]
-            const int j2 = i / (nb * na), k2 = (i / na) % nb, l1 = i % na;
-            double v = 0.0;
-            for (int k1 = 0; k1 < na; k1++) v = fma(A[k2 * na + k1], Y[j2 * a2 + k1 * na + l1], v);
-            X[i] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kOutPerThread; r++) {  // sum_l1 A[l2][l1] X[j2][k2][l1] into this thread's outputs
-            const int i = threadIdx.x + r * TB;
-            if (i < Pb) {
-                const int jk = i / nb, l2 = i % nb;
-                double v = 0.0;
-                for (int l1 = 0; l1 < na; l1++) v = fma(A[l2 * na + l1], X[jk * na + l1], v);
-                acc[r] += v;
-            }
-        }
-        __syncthreads();  // X is refilled by the next degree
+        const int na = ds.na[k];
+        degree_add_dispatch<2, NB>(na, ds.in[k] + b * (size_t)(na * na * na), sA[k], X, Y, acc);
     }
     if (!any) return;
 #pragma unroll
-    for (int r = 0; r < kOutPerThread; r++) {
+    for (int r = 0; r < R; r++) {
         const int i = threadIdx.x + r * TB;
         if (i < Pb) out[b * Pb + i] += acc[r];
     }
@@ -645,7 +673,14 @@ void downward_all(const btk_run_config &cfg, const btk_options &o, const List &l
             ds.goff[di] = l.goff.p + (size_t)(l.ndeg + di) * t.pc;
             ds.na[di] = d + 1;
         }
-        k_degree_add<<<C * t.pc, TB, 0, s>>>(ds, n + 1, Pn, t.pc);
+        switch (n + 1) {
+#define BTK_DEGREE_ADD(N)                                                                                       \
+    case N: k_degree_add<N><<<C * t.pc, TB, 0, s>>>(ds, Pn, t.pc); break;
+            BTK_DEGREE_ADD(3) BTK_DEGREE_ADD(4) BTK_DEGREE_ADD(5) BTK_DEGREE_ADD(6) BTK_DEGREE_ADD(7)
+            BTK_DEGREE_ADD(8) BTK_DEGREE_ADD(9) BTK_DEGREE_ADD(10) BTK_DEGREE_ADD(11)
+#undef BTK_DEGREE_ADD
+            default: throw Error("interp_degree must be between 2 and 10 with several degrees");
+        }
         BTK_LAUNCH_CHECK();
     }
     downward_pass(cfg, o, t, Pn, out, s, sorted_out);
