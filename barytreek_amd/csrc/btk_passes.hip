@@ -129,14 +129,33 @@ __device__ void interp_matrices(const Boxes &bx, int parent, int child, int n1, 
 constexpr int kMaxP = kMaxN1 * kMaxN1 * kMaxN1;
 constexpr int kOutPerThread = (kMaxP + TB - 1) / TB;
 
+// Calls f(std::integral_constant<int, n1>{}) for n1 = 2 .. kMaxN1.
+template <typename F>
+void launch_by_n1(int n1, F &&f) {
+    switch (n1) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 9: f(std::integral_constant<int, 9>{}); break;
+        case 10: f(std::integral_constant<int, 10>{}); break;
+        case 11: f(std::integral_constant<int, 11>{}); break;
+        default: throw Error("interp_degree must be between 1 and 10");
+    }
+}
+
 // ---------------------------------------------------------------------------
 // child -> parent (upward_pass_impl.hpp:170-213): one workgroup per parent.
 // ---------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, int N1>
 __global__ void __launch_bounds__(TB)
-    k_m2m(int ls, const int *__restrict__ child0, const int *__restrict__ count, Boxes bx, int n1, NodeTables tabs,
+    k_m2m(int ls, const int *__restrict__ child0, const int *__restrict__ count, Boxes bx, NodeTables tabs,
           double *__restrict__ W) {
     constexpr int NC = DIM == 3 ? 8 : 4;
+    constexpr int n1 = N1;  // compile-time: index arithmetic by constants, inner loops unrolled
     const int p = ls + blockIdx.x;
     const int c0 = child0[p];
     if (c0 < 0) return;
@@ -210,10 +229,11 @@ __global__ void __launch_bounds__(TB)
 // parent -> child (downward_pass_impl.hpp:118-190): one workgroup per child.
 // Psi is [C][pc][P]; the child's values are added to.
 // ---------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, int N1>
 __global__ void __launch_bounds__(TB)
-    k_l2l(int ls, const int *__restrict__ parent, Boxes bx, int n1, NodeTables tabs, int ncomp, size_t comp_stride,
+    k_l2l(int ls, const int *__restrict__ parent, Boxes bx, NodeTables tabs, int ncomp, size_t comp_stride,
           double *__restrict__ Psi) {
+    constexpr int n1 = N1;  // compile-time: index arithmetic by constants, inner loops unrolled
     const int ch = ls + blockIdx.x;
     const int p = parent[ch];
     const int P = DIM == 3 ? n1 * n1 * n1 : n1 * n1;
@@ -581,9 +601,13 @@ void upward_pass(const btk_run_config &cfg, const Tree &t, const double *q_sorte
     for (int lev = t.levels - 2; lev >= 0; lev--) {
         const int ls = t.level_start[lev], np = t.level_start[lev + 1] - ls;
         if (t.dim == 3)
-            k_m2m<3><<<np, TB, 0, s>>>(ls, t.child0.p, t.count.p, boxes(t), n1, tabs, W);
+            launch_by_n1(n1, [&](auto N) {
+                k_m2m<3, decltype(N)::value><<<np, TB, 0, s>>>(ls, t.child0.p, t.count.p, boxes(t), tabs, W);
+            });
         else
-            k_m2m<2><<<np, TB, 0, s>>>(ls, t.child0.p, t.count.p, boxes(t), n1, tabs, W);
+            launch_by_n1(n1, [&](auto N) {
+                k_m2m<2, decltype(N)::value><<<np, TB, 0, s>>>(ls, t.child0.p, t.count.p, boxes(t), tabs, W);
+            });
         BTK_LAUNCH_CHECK();
     }
 }
@@ -601,9 +625,13 @@ void downward_pass(const btk_run_config &cfg, const btk_options &o, const Tree &
     for (int lev = 0; lev + 1 < t.levels; lev++) {
         const int ls = t.level_start[lev + 1], np = t.level_start[lev + 2] - ls;
         if (t.dim == 3)
-            k_l2l<3><<<np, TB, 0, s>>>(ls, t.parent.p, boxes(t), n1, tabs, ncomp, stride, Psi);
+            launch_by_n1(n1, [&](auto N) {
+                k_l2l<3, decltype(N)::value><<<np, TB, 0, s>>>(ls, t.parent.p, boxes(t), tabs, ncomp, stride, Psi);
+            });
         else
-            k_l2l<2><<<np, TB, 0, s>>>(ls, t.parent.p, boxes(t), n1, tabs, ncomp, stride, Psi);
+            launch_by_n1(n1, [&](auto N) {
+                k_l2l<2, decltype(N)::value><<<np, TB, 0, s>>>(ls, t.parent.p, boxes(t), tabs, ncomp, stride, Psi);
+            });
         BTK_LAUNCH_CHECK();
     }
     // leaf_to_point: every leaf, or only the deepest level's (downward_pass.cpp:26-28)
