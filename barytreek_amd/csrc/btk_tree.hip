@@ -390,17 +390,22 @@ __global__ void __launch_bounds__(TB)
     }
 }
 
+// The scatter of level L writes the refined panels' points into the buffer
+// that level L+1 reads (the two buffers alternate), so a leaf's points stay
+// where the level that created it wrote them: the caller-visible arrays for
+// leaves of even levels, the second buffer for leaves of odd levels.  This
+// pass copies the latter back, once, at the end of the build: one launch per
+// odd level (panels are numbered level by level), one workgroup per panel,
+// leaves only.
 template <int DIM>
 __global__ void __launch_bounds__(TB)
-    k_copy_back(const int *__restrict__ chunk_panel, const int *__restrict__ chunk_j, const int *__restrict__ count,
-                const int *__restrict__ start, const int *__restrict__ perm2, const double *__restrict__ sx2,
-                const double *__restrict__ sy2, const double *__restrict__ sz2, int *__restrict__ perm,
-                double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz) {
-    const int b = blockIdx.x;
-    const int p = chunk_panel[b];
-    const int j = chunk_j[b];
-    const int rs = start[p] + j * CHUNK;
-    const int re = min(start[p] + count[p], rs + CHUNK);
+    k_odd_leaves_back(int ls, const int *__restrict__ child0, const int *__restrict__ count,
+                      const int *__restrict__ start, const int *__restrict__ perm2, const double *__restrict__ sx2,
+                      const double *__restrict__ sy2, const double *__restrict__ sz2, int *__restrict__ perm,
+                      double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz) {
+    const int p = ls + blockIdx.x;
+    if (child0[p] >= 0) return;
+    const int rs = start[p], re = rs + count[p];
     for (int i = rs + threadIdx.x; i < re; i += TB) {
         perm[i] = perm2[i];
         sx[i] = sx2[i];
@@ -529,18 +534,20 @@ void build_levels(Tree &t, hipStream_t s) {
                                                  t.lo[2].p, t.hi[0].p, t.hi[1].p, t.hi[2].p, t.count.p, t.start.p,
                                                  t.parent.p, t.child0.p, t.level.p, chunk_panel.p, chunk_j.p);
         BTK_LAUNCH_CHECK();
+        // this level's points: the caller-visible arrays on even levels, the
+        // second buffer on odd ones (k_odd_leaves_back)
+        int *cp = (lvl & 1) ? perm2.p : t.perm.p, *np_ = (lvl & 1) ? t.perm.p : perm2.p;
+        double *cx = (lvl & 1) ? sx2.p : t.sx.p, *nx = (lvl & 1) ? t.sx.p : sx2.p;
+        double *cy = (lvl & 1) ? sy2.p : t.sy.p, *ny = (lvl & 1) ? t.sy.p : sy2.p;
+        double *cz = DIM == 3 ? ((lvl & 1) ? sz2.p : t.sz.p) : t.sz.p;
+        double *nz = DIM == 3 ? ((lvl & 1) ? t.sz.p : sz2.p) : sz2.p;
         k_classify<DIM><<<T, TB, 0, s>>>(chunk_panel.p, chunk_j.p, t.lo[0].p, t.lo[1].p, t.lo[2].p, t.hi[0].p,
-                                          t.hi[1].p, t.hi[2].p, t.count.p, t.start.p, t.sx.p, t.sy.p, t.sz.p,
-                                          oct.p, hist.p);
+                                          t.hi[1].p, t.hi[2].p, t.count.p, t.start.p, cx, cy, cz, oct.p, hist.p);
         BTK_LAUNCH_CHECK();
         k_panel_scan<NC><<<np, TB, 0, s>>>(ls, val.p, inc.p, hist.p, off.p, t.child0.p, t.count.p, t.start.p);
         BTK_LAUNCH_CHECK();
         k_scatter<DIM><<<T, TB, 0, s>>>(chunk_panel.p, chunk_j.p, t.count.p, t.start.p, t.child0.p, off.p, oct.p,
-                                         t.perm.p, t.sx.p, t.sy.p, t.sz.p, perm2.p, sx2.p, sy2.p, sz2.p,
-                                         t.point_leaf.p);
-        BTK_LAUNCH_CHECK();
-        k_copy_back<DIM><<<T, TB, 0, s>>>(chunk_panel.p, chunk_j.p, t.count.p, t.start.p, perm2.p, sx2.p, sy2.p,
-                                           sz2.p, t.perm.p, t.sx.p, t.sy.p, t.sz.p);
+                                         cp, cx, cy, cz, np_, nx, ny, nz, t.point_leaf.p);
         BTK_LAUNCH_CHECK();
         t.level_start.push_back(t.pc);
         t.pc = pc_new;
@@ -548,6 +555,14 @@ void build_levels(Tree &t, hipStream_t s) {
     }
     t.levels = lvl + 1;
     t.level_start.push_back(t.pc);
+    // leaves of odd levels: their points are in the second buffer
+    for (int l = 1; l < t.levels; l += 2) {
+        const int ls = t.level_start[l], np = t.level_start[l + 1] - ls;
+        if (np == 0) continue;
+        k_odd_leaves_back<DIM><<<np, TB, 0, s>>>(ls, t.child0.p, t.count.p, t.start.p, perm2.p, sx2.p, sy2.p,
+                                                  sz2.p, t.perm.p, t.sx.p, t.sy.p, t.sz.p);
+        BTK_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace
