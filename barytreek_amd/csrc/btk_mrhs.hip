@@ -562,18 +562,18 @@ __global__ void k_weights_f16(int pc, int P, int Ps, const double *__restrict__ 
 // proxy weights (split rows from k_weights_f16, staged in LDS one 64-source
 // tile at a time, the next tile's loads in flight), the B operand the kernel
 // of the key's first pair (FP32 coordinates relative to its target box
-// centre, as the far kernel's TileRel), lane l evaluating it for target l%32
-// and sources 8*(l/32) .. +8 of the step.  The fp16 products overlap the
-// kernel evaluations on the vector ALU.
+// centre, as the far kernel's TileRel, scaled by a power of two), lane l
+// evaluating it for target l%32 and sources 8*(l/32) .. +8 of the step.  The
+// fp16 products overlap the kernel evaluations on the vector ALU.
 // The matrix cores sum a pair's sources in FP32 (3 * Pc / 16 accumulations,
 // 138 at degree 8); each pair's sums, unscaled, are stored to the pair's own
 // FP32 rows of `stage` (Pst floats per component, 128-byte aligned), and
 // k_cc_reduce adds the pairs of each target cluster in FP64.  Each
 // accumulator row is one pair, its lanes 32 consecutive target proxies, so
 // the stores are coalesced.  (FP64 atomic adds into the clusters' sums, one
-// per pair, proxy and component, ran at about 4e11 per second here: on the
-// 16M Plummer solve they cost 19 ms against 6 ms for the stores and the
-// reduction.)
+// per pair, proxy and component, ran at about 4e11 per second: 19 ms of the
+// 16M Plummer solve, where the stores run under the products and the
+// reduction takes 7 ms.)
 struct CcWeights {
     const unsigned short *hi, *lo;  // [source panel][Ps] split-FP16 weights of one degree
     const int *exp;                 // [source panel] their power-of-two scale
